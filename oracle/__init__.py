@@ -1,0 +1,31 @@
+"""CPU oracle for the CP-ALS / MTTKRP path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg use
+this package, and only as the checker (or the timed CPU baseline).  The product
+package ``tenrso_amd`` never imports it.
+
+It wraps ``liboracle.so`` (built from ``cp_oracle.cpp``, a line-faithful C++
+restatement of the reference Rust functions cited there) plus numpy helpers.
+
+Parity status of each function (how it is pinned, see tests/test_oracle.py):
+  mttkrp (faithful f64/f32, fast f64)  pinned: reference KATs (mttkrp.rs:1000-1043
+                                       brute force, property_tests.rs:208-229 all-ones),
+                                       numpy einsum brute force, rank-1 closed form,
+                                       linearity, faithful == fast == blocked-parallel.
+  khatri_rao                           pinned: khatri_rao.rs doc test + tests 479-554.
+  gram / gram_hadamard                 pinned: cp/tests.rs:26-36 KAT, symmetry, einsum.
+  solve_ls                             pinned by the normal equations (A G = M) and
+                                       numpy.linalg.lstsq; scirs2_linalg::lstsq itself
+                                       is third-party and absent -> its rounding and
+                                       failure condition are parity unpinned.
+  fit / recon_norm_sq / inner product  pinned: explicit dense 1-||X-Xhat||/||X||.
+  cp_als (f64, f32, constrained)       pinned: exact low-rank recovery (fit -> 1),
+                                       normal-equation residual of every update,
+                                       convergence-rule unit cases.  orthogonal=True
+                                       uses Householder QR: parity unpinned.
+  cp_reconstruct / extract_weights     pinned: numpy einsum, KAT outer.rs doc test.
+  cp_factor_grads                      pinned: FP64 central differences, identity
+                                       dL/dA_n = A_n G_n - M_n.
+  coo_mttkrp                           pinned: COO -> dense + dense MTTKRP.
+"""
+from .oracle import *  # noqa: F401,F403

@@ -1,0 +1,272 @@
+"""ctypes loader + numpy wrappers for liboracle.so (TEST INFRASTRUCTURE ONLY)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+__all__ = [
+    "lib", "mttkrp", "mttkrp_fast", "mttkrp_blocked_parallel", "khatri_rao", "gram", "gram_hadamard",
+    "solve_ls", "tikhonov", "soft_threshold", "orthonormalize", "norm_sq", "recon_norm_sq", "fit",
+    "cp_als", "cp_reconstruct", "extract_weights", "cp_factor_grads", "coo_mttkrp", "num_threads",
+    "OrcConstraints", "REASONS",
+]
+
+REASONS = ["FitTolerance", "MaxIterations", "Oscillation", "TimeLimit"]
+
+_lib = None
+
+
+class OrcConstraints(C.Structure):
+    _fields_ = [("nonnegative", C.c_int), ("orthogonal", C.c_int), ("l2", C.c_double), ("l1", C.c_double),
+                ("tikhonov_order", C.c_int), ("tikhonov_lambda", C.c_double)]
+
+
+class OrcResult(C.Structure):
+    _fields_ = [("iters", C.c_int), ("reason", C.c_int), ("oscillation_count", C.c_int), ("fit", C.c_double),
+                ("final_fit_change", C.c_double), ("solve_path_max", C.c_int), ("seconds", C.c_double)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            from . import build_oracle
+            build_oracle.build()
+        _lib = C.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _shape(shape):
+    return (C.c_int64 * len(shape))(*shape)
+
+
+def _ptrs(arrs):
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def _prep(X, factors, dtype):
+    X = np.ascontiguousarray(X, dtype=dtype)
+    F = [np.ascontiguousarray(f, dtype=dtype) for f in factors]
+    return X, F
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def mttkrp(X, factors, mode, dtype=np.float64):
+    """Faithful restatement of tenrso_kernels::mttkrp (mttkrp.rs:74-146)."""
+    X, F = _prep(X, factors, dtype)
+    R = F[0].shape[1]
+    out = np.zeros((X.shape[mode], R), dtype=dtype)
+    fn = lib().orc_mttkrp_f64 if dtype == np.float64 else lib().orc_mttkrp_f32
+    rc = fn(_p(X), X.ndim, _shape(X.shape), _ptrs(F), C.c_int64(R), mode, _p(out))
+    if rc:
+        raise ValueError(f"oracle mttkrp failed rc={rc}")
+    return out
+
+
+def mttkrp_fast(X, factors, mode):
+    """Cache-friendly FP64 MTTKRP (OpenMP); same sum as mttkrp.rs:74-146."""
+    X, F = _prep(X, factors, np.float64)
+    R = F[0].shape[1]
+    out = np.zeros((X.shape[mode], R), dtype=np.float64)
+    rc = lib().orc_mttkrp_fast_f64(_p(X), X.ndim, _shape(X.shape), _ptrs(F), C.c_int64(R), mode, _p(out))
+    if rc:
+        raise ValueError(f"oracle mttkrp_fast failed rc={rc}")
+    return out
+
+
+def mttkrp_blocked_parallel(X, factors, mode, tile=64):
+    """OpenMP restatement of mttkrp_blocked_parallel (mttkrp.rs:528-609), f32."""
+    X, F = _prep(X, factors, np.float32)
+    R = F[0].shape[1]
+    out = np.zeros((X.shape[mode], R), dtype=np.float32)
+    rc = lib().orc_mttkrp_blocked_parallel_f32(_p(X), X.ndim, _shape(X.shape), _ptrs(F), C.c_int64(R), mode,
+                                               C.c_int64(tile), _p(out))
+    if rc:
+        raise ValueError(f"oracle mttkrp_blocked_parallel failed rc={rc}")
+    return out
+
+
+def khatri_rao(A, B, dtype=np.float64):
+    """khatri_rao (khatri_rao.rs:56-89)."""
+    A = np.ascontiguousarray(A, dtype=dtype)
+    B = np.ascontiguousarray(B, dtype=dtype)
+    assert A.shape[1] == B.shape[1]
+    out = np.zeros((A.shape[0] * B.shape[0], A.shape[1]), dtype=dtype)
+    fn = lib().orc_khatri_rao_f64 if dtype == np.float64 else lib().orc_khatri_rao_f32
+    fn(_p(A), C.c_int64(A.shape[0]), _p(B), C.c_int64(B.shape[0]), C.c_int64(A.shape[1]), _p(out))
+    return out
+
+
+def gram(Fm):
+    """compute_gram_matrix (cp/helpers.rs:427-452)."""
+    Fm = np.ascontiguousarray(Fm, dtype=np.float64)
+    G = np.zeros((Fm.shape[1], Fm.shape[1]))
+    lib().orc_gram_f64(_p(Fm), C.c_int64(Fm.shape[0]), C.c_int64(Fm.shape[1]), _p(G))
+    return G
+
+
+def gram_hadamard(factors, skip):
+    """compute_gram_hadamard (cp/helpers.rs:409-424)."""
+    F = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    R = F[0].shape[1]
+    out = np.zeros((R, R))
+    shape = [f.shape[0] for f in F]
+    lib().orc_gram_hadamard_f64(_ptrs(F), len(F), _shape(shape), C.c_int64(R), skip, _p(out))
+    return out
+
+
+def solve_ls(M, G):
+    """solve_least_squares (cp/helpers.rs:455-495); returns (X, path)."""
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    G = np.ascontiguousarray(G, dtype=np.float64)
+    out = np.zeros_like(M)
+    path = lib().orc_solve_ls_f64(_p(M), C.c_int64(M.shape[0]), C.c_int64(M.shape[1]), _p(G), _p(out))
+    return out, path
+
+
+def tikhonov(G, lam, order):
+    """apply_tikhonov_to_gram (cp/helpers.rs:630-702)."""
+    G = np.array(G, dtype=np.float64, order="C")
+    lib().orc_tikhonov_f64(_p(G), C.c_int64(G.shape[0]), C.c_double(lam), C.c_int64(order))
+    return G
+
+
+def soft_threshold(Fm, th):
+    """soft_threshold (cp/helpers.rs:610-623)."""
+    Fm = np.array(Fm, dtype=np.float64, order="C")
+    lib().orc_soft_threshold_f64(_p(Fm), C.c_int64(Fm.size), C.c_double(th))
+    return Fm
+
+
+def orthonormalize(Fm):
+    Fm = np.array(Fm, dtype=np.float64, order="C")
+    lib().orc_orthonormalize_f64(_p(Fm), C.c_int64(Fm.shape[0]), C.c_int64(Fm.shape[1]))
+    return Fm
+
+
+lib_restype_done = False
+
+
+def _restypes():
+    global lib_restype_done
+    if not lib_restype_done:
+        L = lib()
+        L.orc_norm_sq_f64.restype = C.c_double
+        L.orc_norm_sq_f32.restype = C.c_float
+        L.orc_recon_norm_sq_f64.restype = C.c_double
+        L.orc_fit_f64.restype = C.c_double
+        lib_restype_done = True
+
+
+def norm_sq(X, dtype=np.float64):
+    """compute_norm_squared (cp/helpers.rs:498-510), sequential in dtype."""
+    _restypes()
+    X = np.ascontiguousarray(X, dtype=dtype)
+    fn = lib().orc_norm_sq_f64 if dtype == np.float64 else lib().orc_norm_sq_f32
+    return float(fn(_p(X), C.c_int64(X.size)))
+
+
+def recon_norm_sq(factors):
+    """compute_reconstruction_norm_squared (cp/helpers.rs:533-555)."""
+    _restypes()
+    F = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    shape = [f.shape[0] for f in F]
+    return float(lib().orc_recon_norm_sq_f64(_ptrs(F), len(F), _shape(shape), C.c_int64(F[0].shape[1])))
+
+
+def fit(X, factors, tensor_norm_sq=None, fast=True):
+    """compute_fit (cp/helpers.rs:513-530) in FP64."""
+    _restypes()
+    X, F = _prep(X, factors, np.float64)
+    if tensor_norm_sq is None:
+        tensor_norm_sq = norm_sq(X)
+    return float(lib().orc_fit_f64(_p(X), X.ndim, _shape(X.shape), _ptrs(F), C.c_int64(F[0].shape[1]),
+                                   C.c_double(tensor_norm_sq), int(fast)))
+
+
+def cp_als(X, init_factors, max_iters, tol, time_limit=None, nonnegative=False, orthogonal=False, l2=0.0,
+           l1=0.0, tikhonov_order=0, tikhonov_lambda=0.0, dtype=np.float64, fast=True):
+    """cp_als / cp_als_constrained (cp/core.rs:59-182, 225-408) from explicit initial factors.
+
+    Returns (factors, info) where info mirrors CpDecomp + ConvergenceInfo.
+    """
+    X, F = _prep(X, init_factors, dtype)
+    F = [f.copy() for f in F]
+    R = F[0].shape[1]
+    cons = OrcConstraints(int(nonnegative), int(orthogonal), float(l2), float(l1), int(tikhonov_order),
+                          float(tikhonov_lambda))
+    res = OrcResult()
+    hist = np.zeros(max(max_iters, 1), dtype=np.float64)
+    tl = -1.0 if time_limit is None else float(time_limit)
+    if dtype == np.float64:
+        rc = lib().orc_cp_als_f64(_p(X), X.ndim, _shape(X.shape), C.c_int64(R), _ptrs(F), max_iters,
+                                  C.c_double(tol), C.c_double(tl), C.byref(cons), int(fast), _p(hist),
+                                  C.byref(res))
+    else:
+        rc = lib().orc_cp_als_f32(_p(X), X.ndim, _shape(X.shape), C.c_int64(R), _ptrs(F), max_iters,
+                                  C.c_double(tol), C.c_double(tl), C.byref(cons), _p(hist), C.byref(res))
+    if rc:
+        raise ValueError({10: "InvalidRank", 11: "InvalidTolerance", 12: "NonnegativeViolation"}.get(rc, rc))
+    info = dict(iters=res.iters, reason=REASONS[res.reason], oscillation_count=res.oscillation_count,
+                oscillated=res.oscillation_count > 0, fit=res.fit, final_fit_change=res.final_fit_change,
+                fit_history=hist[:res.iters].copy(), solve_path_max=res.solve_path_max, seconds=res.seconds)
+    return F, info
+
+
+def cp_reconstruct(factors, weights=None):
+    """cp_reconstruct (tenrso-kernels/src/outer.rs:223-274)."""
+    F = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    shape = [f.shape[0] for f in F]
+    out = np.zeros(shape, dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    lib().orc_cp_reconstruct_f64(_ptrs(F), len(F), _shape(shape), C.c_int64(F[0].shape[1]),
+                                 None if w is None else _p(w), _p(out))
+    return out
+
+
+def extract_weights(factors):
+    """CpDecomp::extract_weights (cp/types.rs:421-446); returns (factors, weights)."""
+    F = [np.array(f, dtype=np.float64, order="C") for f in factors]
+    shape = [f.shape[0] for f in F]
+    R = F[0].shape[1]
+    w = np.zeros(R)
+    lib().orc_extract_weights_f64(_ptrs(F), len(F), _shape(shape), C.c_int64(R), _p(w))
+    return F, w
+
+
+def cp_factor_grads(grad_output, factors, weights=None):
+    """CpReconstructionGrad::compute_factor_gradients (tenrso-ad/src/grad.rs:81-120)."""
+    G, F = _prep(grad_output, factors, np.float64)
+    R = F[0].shape[1]
+    outs = [np.zeros((G.shape[k], R)) for k in range(G.ndim)]
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    rc = lib().orc_cp_factor_grads_f64(_p(G), G.ndim, _shape(G.shape), _ptrs(F), C.c_int64(R),
+                                       None if w is None else _p(w), _ptrs(outs))
+    if rc:
+        raise ValueError(f"oracle cp_factor_grads failed rc={rc}")
+    return outs
+
+
+def coo_mttkrp(indices, values, shape, factors, mode):
+    """FP64 COO MTTKRP: dense semantics of CooTensor::to_dense (coo.rs:267-286) + mttkrp.
+
+    indices: (ndim, nnz) int32 SoA; values: (nnz,) float32.
+    """
+    idx = np.ascontiguousarray(indices, dtype=np.int32)
+    vals = np.ascontiguousarray(values, dtype=np.float32)
+    F = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+    R = F[0].shape[1]
+    out = np.zeros((shape[mode], R))
+    lib().orc_coo_mttkrp_f64(_p(idx), _p(vals), C.c_int64(vals.size), len(shape), _shape(shape), _ptrs(F),
+                             C.c_int64(R), mode, _p(out))
+    return out
