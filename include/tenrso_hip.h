@@ -1,0 +1,189 @@
+/*
+ * tenrso_hip.h -- C ABI of the MI355X (gfx950) CP decomposition hot path.
+ *
+ * Drop-in boundary for tenrso's CP path.  Every entry point below names the
+ * reference function it replaces (paths relative to the tenrso repository).
+ *
+ * Conventions
+ *   - Tensors are dense, C-contiguous (row-major, last mode fastest), like
+ *     DenseND<T> (tenrso-core/src/dense/types.rs:40-43).  Factor matrices are
+ *     row-major I_k x R, like ndarray Array2 (standard layout).
+ *   - Functions without the `_host` suffix take DEVICE pointers (hipMalloc'd or
+ *     torch.cuda tensors); `shape` and the `factors` pointer arrays themselves are
+ *     host arrays.  `_host` variants take host pointers and stage through the
+ *     context's device buffers (what a Rust `extern "C"` shim over ArrayView calls).
+ *   - Every call returns a status (TENRSO_OK == 0).  On failure the message that
+ *     the reference would put in its anyhow/CpError is available from
+ *     tenrso_last_error() (thread-local).  Nothing panics or throws across the ABI.
+ *   - Work is enqueued on the context's stream.  Device-pointer calls are
+ *     asynchronous unless stated; `_host` calls and cp_als synchronise.
+ *   - A context serialises its calls; distinct contexts may be used from distinct
+ *     threads.
+ */
+#ifndef TENRSO_HIP_H
+#define TENRSO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: CpError (tenrso-decomp/src/cp/types.rs:13-35) and
+ *      KernelError / anyhow messages (tenrso-kernels/src/mttkrp.rs:85-122) ---- */
+enum tenrso_status {
+  TENRSO_OK = 0,
+  TENRSO_ERR_INVALID_ARGUMENT = 1,       /* mttkrp/khatri_rao validation (anyhow::bail!) */
+  TENRSO_ERR_INVALID_RANK = 2,           /* CpError::InvalidRank */
+  TENRSO_ERR_INVALID_TOLERANCE = 3,      /* CpError::InvalidTolerance */
+  TENRSO_ERR_SHAPE_MISMATCH = 4,         /* CpError::ShapeMismatch */
+  TENRSO_ERR_LINALG = 5,                 /* CpError::LinalgError */
+  TENRSO_ERR_NONNEGATIVE = 6,            /* CpError::NonnegativeViolation */
+  TENRSO_ERR_INVALID_REGULARIZATION = 7, /* CpError::InvalidRegularization */
+  TENRSO_ERR_DEVICE = 8,                 /* HIP runtime failure */
+  TENRSO_ERR_UNSUPPORTED = 9
+};
+
+/* precision of the MTTKRP inside cp_als */
+enum tenrso_precision {
+  TENRSO_PREC_FP32 = 0, /* plain FP32 MFMA MTTKRP: the drop-in */
+  TENRSO_PREC_BF16 = 1  /* BF16-input / FP32-accumulate MFMA MTTKRP */
+};
+
+/* ConvergenceReason (tenrso-decomp/src/cp/types.rs:312-322) */
+enum tenrso_reason {
+  TENRSO_REASON_FIT_TOLERANCE = 0,
+  TENRSO_REASON_MAX_ITERATIONS = 1,
+  TENRSO_REASON_OSCILLATION = 2,
+  TENRSO_REASON_TIME_LIMIT = 3
+};
+
+typedef struct tenrso_ctx tenrso_ctx;
+
+/* Thread-local message of the last failing call on this thread ("" if none). */
+const char* tenrso_last_error(void);
+/* Library version string. */
+const char* tenrso_version(void);
+
+/* Context: owns a HIP stream, or borrows `stream` when borrow_stream != 0
+ * (NULL then means the legacy default stream, e.g. torch's default stream), plus
+ * device scratch and timing events.  No reference counterpart: the reference is
+ * in-process CPU code. */
+int tenrso_ctx_create(int device, int borrow_stream, void* stream, tenrso_ctx** out);
+int tenrso_ctx_destroy(tenrso_ctx* ctx);
+int tenrso_ctx_synchronize(tenrso_ctx* ctx);
+/* When timing is enabled, MTTKRP entry points record HIP events around their
+ * kernels and synchronise; tenrso_ctx_last_kernel_ms returns the last duration. */
+int tenrso_ctx_set_timing(tenrso_ctx* ctx, int enabled);
+int tenrso_ctx_last_kernel_ms(tenrso_ctx* ctx, double* ms);
+
+/* ------------------------------------------------------------------------
+ * Dense MTTKRP.  Replaces tenrso_kernels::mttkrp (tenrso-kernels/src/mttkrp.rs:74-146)
+ * and, with the same contract, mttkrp_blocked / _blocked_parallel / _fused /
+ * _fused_parallel (mttkrp.rs:395-498, 528-609, 783-878, 903-992).
+ *   out[i, r] = sum_{i_k, k != mode} X[..., i, ...] * prod_{k != mode} F_k[i_k, r]
+ * X: prod(shape) floats; factors[k]: shape[k] x rank; out: shape[mode] x rank.
+ * Khatri-Rao rows are formed on the fly; no unfolding copy is made.
+ * ------------------------------------------------------------------------ */
+int tenrso_mttkrp_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                      const float* const* factors, int64_t rank, int mode, float* out);
+/* Argument check with the reference's messages and order (mttkrp.rs:85-122,
+ * 194-196), for callers whose factors may disagree in shape: factor_rows[k] x
+ * factor_cols[k] is the shape of factor k.  No device work. */
+int tenrso_mttkrp_check(int ndim, const int64_t* shape, int nfactors, const int64_t* factor_rows,
+                        const int64_t* factor_cols, int mode);
+int tenrso_mttkrp_f32_host(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                           const float* const* factors, int64_t rank, int mode, float* out);
+/* Same contract with FP64 accumulation of the split partial sums into a double
+ * output (used by cp_als; exposed for precision studies). */
+int tenrso_mttkrp_f32_f64out(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                             const float* const* factors, int64_t rank, int mode, double* out);
+
+/* BF16-input / FP32-accumulate MFMA MTTKRP.  X_bf16 is the tensor rounded to
+ * bfloat16 (RNE, tenrso_convert_f32_to_bf16); factors stay FP32 and the
+ * Khatri-Rao rows are formed in FP32 and rounded once to bf16.  No reference
+ * counterpart (the reference has no reduced-precision MTTKRP). */
+int tenrso_mttkrp_bf16(tenrso_ctx* ctx, const uint16_t* X_bf16, int ndim, const int64_t* shape,
+                       const float* const* factors, int64_t rank, int mode, float* out);
+int tenrso_convert_f32_to_bf16(tenrso_ctx* ctx, const float* src, int64_t n, uint16_t* dst);
+
+/* mttkrp_all_modes (tenrso-kernels/src/utils.rs:282-311): outs[k] for every mode. */
+int tenrso_mttkrp_all_modes_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                                const float* const* factors, int64_t rank, float* const* outs);
+
+/* ------------------------------------------------------------------------
+ * Khatri-Rao product.  Replaces tenrso_kernels::khatri_rao (khatri_rao.rs:56-89)
+ * and its _parallel / _blocked / _blocked_parallel variants (117-357), which are
+ * bit-identical: C[ia*J + ib, k] = A[ia, k] * B[ib, k].  The reference panics on
+ * a column mismatch; here the caller passes one K, and K/I/J <= 0 is
+ * TENRSO_ERR_INVALID_ARGUMENT.
+ * ------------------------------------------------------------------------ */
+int tenrso_khatri_rao_f32(tenrso_ctx* ctx, const float* A, int64_t I, const float* B, int64_t J, int64_t K,
+                          float* C);
+int tenrso_khatri_rao_f32_host(tenrso_ctx* ctx, const float* A, int64_t I, const float* B, int64_t J,
+                               int64_t K, float* C);
+
+/* ------------------------------------------------------------------------
+ * CP reconstruction.  Replaces tenrso_kernels::cp_reconstruct (outer.rs:223-274)
+ * used by CpDecomp::reconstruct (cp/types.rs:384-415):
+ *   out[i_0..i_{N-1}] = sum_r w_r prod_k F_k[i_k, r]   (weights may be NULL = 1).
+ * ------------------------------------------------------------------------ */
+int tenrso_cp_reconstruct_f32(tenrso_ctx* ctx, int ndim, const int64_t* shape, const float* const* factors,
+                              int64_t rank, const float* weights, float* out);
+
+/* ------------------------------------------------------------------------
+ * CP-ALS.  Replaces tenrso_decomp::cp_als (cp/core.rs:59-182) and
+ * cp_als_constrained (cp/core.rs:225-408) for T = f32, with the initial
+ * factors supplied by the caller (the reference draws them from an unseeded
+ * thread_rng, cp/helpers.rs:212-231; a shim draws them and passes them here).
+ * Per iteration: for each mode, MTTKRP -> Gram-Hadamard (FP64) -> Cholesky
+ * solve (FP64, ridge retry, eigen pseudo-inverse fallback) -> constraints; then
+ * the fit (FP64).  Convergence, oscillation and time-limit rules are the
+ * reference's (core.rs:117-165).
+ * ------------------------------------------------------------------------ */
+typedef struct tenrso_cp_options {
+  int64_t rank;
+  int32_t max_iters;
+  int32_t precision;             /* enum tenrso_precision */
+  double tol;                    /* in [0, 1) */
+  double time_limit_s;           /* < 0: None */
+  /* CpConstraints (cp/types.rs:198-309), already reduced to effective values */
+  int32_t nonnegative;
+  int32_t validate_nonneg_input; /* default 1 */
+  int32_t orthogonal;
+  int32_t tikhonov_order;        /* > 0: apply_tikhonov_to_gram (cp/helpers.rs:630-702) */
+  double l2;                     /* effective_l2() */
+  double l1;                     /* effective_l1() */
+  double tikhonov_lambda;
+  int32_t fit_from_mode0;        /* 1: separate mode-0 MTTKRP for <X, Xhat> as in
+                                    cp/helpers.rs:558-579; 0 (default): reuse the
+                                    last mode's MTTKRP (same value in exact arithmetic) */
+  int32_t reserved;
+} tenrso_cp_options;
+
+typedef struct tenrso_cp_result {
+  int32_t iters;
+  int32_t reason;                /* enum tenrso_reason */
+  int32_t oscillation_count;
+  int32_t oscillated;
+  double fit;
+  double final_fit_change;
+  int32_t solve_path_max;        /* 0 Cholesky, 1 ridge retry, 2 pseudo-inverse */
+  int32_t reserved;
+  double seconds;                /* wall time of the ALS loop */
+} tenrso_cp_result;
+
+/* X: device tensor; factors: device pointers, in = initial factors, out = result.
+ * fit_history: host buffer of max_iters doubles, or NULL. */
+int tenrso_cp_als_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, float* const* factors,
+                      const tenrso_cp_options* opts, tenrso_cp_result* result, double* fit_history);
+int tenrso_cp_als_f32_host(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                           float* const* factors, const tenrso_cp_options* opts, tenrso_cp_result* result,
+                           double* fit_history);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TENRSO_HIP_H */

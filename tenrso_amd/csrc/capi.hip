@@ -1,0 +1,308 @@
+// capi.hip -- extern "C" entry points (include/tenrso_hip.h).
+//
+// Argument validation reproduces the reference's checks and messages:
+//   mttkrp      tenrso-kernels/src/mttkrp.rs:85-122 (+ 194-196 for < 2 factors)
+//   cp_als      tenrso-decomp/src/cp/core.rs:82-95 (CpError Display strings, cp/types.rs:13-35)
+#include <cstring>
+#include <vector>
+
+#include "cp_als.hpp"
+#include "kr_reconstruct.hpp"
+#include "mttkrp.hpp"
+
+using namespace tenrso;
+
+namespace {
+
+bool bad_ctx(tenrso_ctx* ctx) {
+  if (!ctx) {
+    set_error("null context");
+    return true;
+  }
+  return false;
+}
+
+i64 prod(const int64_t* s, int n) {
+  i64 p = 1;
+  for (int k = 0; k < n; ++k) p *= s[k];
+  return p;
+}
+
+int check_mttkrp_basic(int ndim, const int64_t* shape, int64_t rank, int mode) {
+  if (ndim <= 0 || !shape) {
+    set_error("tensor must have at least one mode");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  if (mode < 0 || mode >= ndim) {
+    set_error("Mode %d out of bounds for tensor with rank %d", mode, ndim);
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  if (ndim > MAXD) {
+    set_error("tensors with more than %d modes are not supported", MAXD);
+    return TENRSO_ERR_UNSUPPORTED;
+  }
+  for (int k = 0; k < ndim; ++k)
+    if (shape[k] < 0) {
+      set_error("negative mode size %lld", (long long)shape[k]);
+      return TENRSO_ERR_INVALID_ARGUMENT;
+    }
+  if (rank < 0) {
+    set_error("negative rank %lld", (long long)rank);
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  if (ndim < 2) {
+    set_error("Need at least 2 factor matrices for MTTKRP");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  return TENRSO_OK;
+}
+
+int timed_begin(tenrso_ctx* ctx) {
+  if (ctx->timing) TENRSO_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  return TENRSO_OK;
+}
+int timed_end(tenrso_ctx* ctx) {
+  if (ctx->timing) {
+    TENRSO_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+    TENRSO_HIP_TRY(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    TENRSO_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_kernel_ms = ms;
+  }
+  return TENRSO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tenrso_mttkrp_check(int ndim, const int64_t* shape, int nfactors, const int64_t* factor_rows,
+                        const int64_t* factor_cols, int mode) {
+  clear_error();
+  if (ndim <= 0 || !shape) {
+    set_error("tensor must have at least one mode");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  if (mode < 0 || mode >= ndim) {
+    set_error("Mode %d out of bounds for tensor with rank %d", mode, ndim);
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  if (nfactors != ndim) {
+    set_error("Number of factor matrices (%d) must match tensor rank (%d)", nfactors, ndim);
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  const int64_t cp_rank = factor_cols[0];
+  for (int i = 0; i < nfactors; ++i) {
+    if (factor_cols[i] != cp_rank) {
+      set_error("Factor matrix %d has %lld columns, expected %lld", i, (long long)factor_cols[i],
+                (long long)cp_rank);
+      return TENRSO_ERR_INVALID_ARGUMENT;
+    }
+    if (factor_rows[i] != shape[i]) {
+      set_error("Factor matrix %d has %lld rows, expected %lld (tensor mode-%d size)", i,
+                (long long)factor_rows[i], (long long)shape[i], i);
+      return TENRSO_ERR_INVALID_ARGUMENT;
+    }
+  }
+  if (ndim < 2) {
+    set_error("Need at least 2 factor matrices for MTTKRP");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  return TENRSO_OK;
+}
+
+int tenrso_mttkrp_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, const float* const* factors,
+                      int64_t rank, int mode, float* out) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  TENRSO_TRY(check_mttkrp_basic(ndim, shape, rank, mode));
+  if (rank == 0 || shape[mode] == 0) return TENRSO_OK;
+  TENRSO_TRY(set_device(ctx));
+  TENRSO_TRY(timed_begin(ctx));
+  TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, X, nullptr, ndim, shape, factors, rank, mode, MttkrpOut::F32, out,
+                          nullptr));
+  return timed_end(ctx);
+}
+
+int tenrso_mttkrp_f32_f64out(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                             const float* const* factors, int64_t rank, int mode, double* out) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  TENRSO_TRY(check_mttkrp_basic(ndim, shape, rank, mode));
+  if (rank == 0 || shape[mode] == 0) return TENRSO_OK;
+  TENRSO_TRY(set_device(ctx));
+  TENRSO_TRY(timed_begin(ctx));
+  TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, X, nullptr, ndim, shape, factors, rank, mode, MttkrpOut::F64, out,
+                          nullptr));
+  return timed_end(ctx);
+}
+
+int tenrso_convert_f32_to_bf16(tenrso_ctx* ctx, const float* src, int64_t n, uint16_t* dst) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  if (n < 0) {
+    set_error("negative length");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  TENRSO_TRY(set_device(ctx));
+  const i64 shape1[1] = {n};
+  return convert_bf16(ctx, src, 1, shape1, false, dst);
+}
+
+int tenrso_mttkrp_bf16(tenrso_ctx* ctx, const uint16_t* X_bf16, int ndim, const int64_t* shape,
+                       const float* const* factors, int64_t rank, int mode, float* out) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  TENRSO_TRY(check_mttkrp_basic(ndim, shape, rank, mode));
+  if (rank == 0 || shape[mode] == 0) return TENRSO_OK;
+  TENRSO_TRY(set_device(ctx));
+  const void* Xt = nullptr;
+  if (mode == ndim - 1) {
+    // last mode: move it to the front once (bf16 -> bf16 transpose)
+    const i64 total = prod(shape, ndim);
+    void* pt = nullptr;
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * 2 + 16, &pt));
+    TENRSO_TRY(transpose_last_to_front_u16(ctx, X_bf16, ndim, shape, (uint16_t*)pt));
+    Xt = pt;
+  }
+  TENRSO_TRY(timed_begin(ctx));
+  TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_BF16, X_bf16, Xt, ndim, shape, factors, rank, mode, MttkrpOut::F32, out,
+                          nullptr));
+  return timed_end(ctx);
+}
+
+int tenrso_mttkrp_all_modes_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                                const float* const* factors, int64_t rank, float* const* outs) {
+  for (int m = 0; m < ndim; ++m) TENRSO_TRY(tenrso_mttkrp_f32(ctx, X, ndim, shape, factors, rank, m, outs[m]));
+  return TENRSO_OK;
+}
+
+int tenrso_mttkrp_f32_host(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                           const float* const* factors, int64_t rank, int mode, float* out) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  TENRSO_TRY(check_mttkrp_basic(ndim, shape, rank, mode));
+  if (rank == 0 || shape[mode] == 0) return TENRSO_OK;
+  TENRSO_TRY(set_device(ctx));
+  const i64 total = prod(shape, ndim);
+  i64 ftot = 0;
+  for (int k = 0; k < ndim; ++k) ftot += shape[k] * rank;
+  void *dx, *df, *dout;
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_STAGE_X, (size_t)total * sizeof(float), &dx));
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_STAGE_F, (size_t)ftot * sizeof(float), &df));
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_STAGE_OUT, (size_t)shape[mode] * rank * sizeof(float), &dout));
+  TENRSO_HIP_TRY(hipMemcpyAsync(dx, X, (size_t)total * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  std::vector<const float*> dF(ndim);
+  i64 off = 0;
+  for (int k = 0; k < ndim; ++k) {
+    float* p = (float*)df + off;
+    TENRSO_HIP_TRY(hipMemcpyAsync(p, factors[k], (size_t)shape[k] * rank * sizeof(float), hipMemcpyHostToDevice,
+                                  ctx->stream));
+    dF[k] = p;
+    off += shape[k] * rank;
+  }
+  TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, dx, nullptr, ndim, shape, dF.data(), rank, mode, MttkrpOut::F32,
+                          dout, nullptr));
+  TENRSO_HIP_TRY(hipMemcpyAsync(out, dout, (size_t)shape[mode] * rank * sizeof(float), hipMemcpyDeviceToHost,
+                                ctx->stream));
+  TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TENRSO_OK;
+}
+
+int tenrso_khatri_rao_f32(tenrso_ctx* ctx, const float* A, int64_t I, const float* B, int64_t J, int64_t K,
+                          float* C) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  if (I < 0 || J < 0 || K < 0) {
+    set_error("Number of columns must match and sizes must be non-negative");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  TENRSO_TRY(set_device(ctx));
+  return khatri_rao_f32(ctx, A, I, B, J, K, C);
+}
+
+int tenrso_khatri_rao_f32_host(tenrso_ctx* ctx, const float* A, int64_t I, const float* B, int64_t J, int64_t K,
+                               float* C) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  if (I < 0 || J < 0 || K < 0) {
+    set_error("Number of columns must match and sizes must be non-negative");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  if (I * J * K == 0) return TENRSO_OK;
+  TENRSO_TRY(set_device(ctx));
+  void *da, *db, *dc;
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_STAGE_X, (size_t)(I + J) * K * sizeof(float), &da));
+  db = (float*)da + I * K;
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_STAGE_OUT, (size_t)I * J * K * sizeof(float), &dc));
+  TENRSO_HIP_TRY(hipMemcpyAsync(da, A, (size_t)I * K * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  TENRSO_HIP_TRY(hipMemcpyAsync(db, B, (size_t)J * K * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  TENRSO_TRY(khatri_rao_f32(ctx, (const float*)da, I, (const float*)db, J, K, (float*)dc));
+  TENRSO_HIP_TRY(hipMemcpyAsync(C, dc, (size_t)I * J * K * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TENRSO_OK;
+}
+
+int tenrso_cp_reconstruct_f32(tenrso_ctx* ctx, int ndim, const int64_t* shape, const float* const* factors,
+                              int64_t rank, const float* weights, float* out) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  if (ndim <= 0) {
+    set_error("Need at least one factor matrix for CP reconstruction");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  if (ndim > MAXD_RECON) {
+    set_error("tensors with more than %d modes are not supported", MAXD_RECON);
+    return TENRSO_ERR_UNSUPPORTED;
+  }
+  TENRSO_TRY(set_device(ctx));
+  return cp_reconstruct_f32(ctx, ndim, shape, factors, rank, weights, out);
+}
+
+int tenrso_cp_als_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, float* const* factors,
+                      const tenrso_cp_options* opts, tenrso_cp_result* result, double* fit_history) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  if (!opts || ndim <= 0 || !shape) {
+    set_error("null options or empty shape");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  return cp_als_device(ctx, X, ndim, shape, factors, opts, result, fit_history);
+}
+
+int tenrso_cp_als_f32_host(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, float* const* factors,
+                           const tenrso_cp_options* opts, tenrso_cp_result* result, double* fit_history) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  if (!opts || ndim <= 0 || !shape || ndim > MAXD) {
+    set_error("null options, empty shape or too many modes");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  TENRSO_TRY(set_device(ctx));
+  const i64 total = prod(shape, ndim);
+  const i64 R = opts->rank;
+  i64 ftot = 0;
+  for (int k = 0; k < ndim; ++k) ftot += shape[k] * (R > 0 ? R : 0);
+  void *dx, *df;
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_STAGE_X, (size_t)total * sizeof(float), &dx));
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_STAGE_F, (size_t)ftot * sizeof(float) + 16, &df));
+  TENRSO_HIP_TRY(hipMemcpyAsync(dx, X, (size_t)total * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  std::vector<float*> dF(ndim);
+  i64 off = 0;
+  for (int k = 0; k < ndim; ++k) {
+    dF[k] = (float*)df + off;
+    if (R > 0)
+      TENRSO_HIP_TRY(hipMemcpyAsync(dF[k], factors[k], (size_t)shape[k] * R * sizeof(float), hipMemcpyHostToDevice,
+                                    ctx->stream));
+    off += shape[k] * (R > 0 ? R : 0);
+  }
+  int rc = cp_als_device(ctx, (const float*)dx, ndim, shape, dF.data(), opts, result, fit_history);
+  if (rc != TENRSO_OK) return rc;
+  for (int k = 0; k < ndim; ++k)
+    TENRSO_HIP_TRY(hipMemcpyAsync(factors[k], dF[k], (size_t)shape[k] * R * sizeof(float), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+  TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TENRSO_OK;
+}
+
+}  // extern "C"

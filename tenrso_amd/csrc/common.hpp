@@ -1,0 +1,90 @@
+// common.hpp -- shared host/device helpers of the gfx950 CP library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/tenrso_hip.h"
+
+namespace tenrso {
+
+using i64 = int64_t;
+
+// ---- error reporting ------------------------------------------------------
+void set_error(const char* fmt, ...);
+void clear_error();
+
+struct Status {
+  int code = TENRSO_OK;
+  bool ok() const { return code == TENRSO_OK; }
+};
+
+#define TENRSO_HIP_TRY(expr)                                                                      \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess) {                                                                       \
+      ::tenrso::set_error("HIP error %s at %s:%d (%s)", hipGetErrorString(e_), __FILE__, __LINE__, \
+                          #expr);                                                                 \
+      return TENRSO_ERR_DEVICE;                                                                   \
+    }                                                                                             \
+  } while (0)
+
+#define TENRSO_TRY(expr)         \
+  do {                           \
+    int rc_ = (expr);            \
+    if (rc_ != TENRSO_OK) return rc_; \
+  } while (0)
+
+// ---- device buffer pool ---------------------------------------------------
+// Grow-only named buffers owned by a context (partials, Grams, staging).
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace tenrso
+
+struct tenrso_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int num_cus = 256;
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_kernel_ms = 0.0;
+  std::vector<tenrso::DevBuf> bufs;  // indexed by slot id
+  void* pinned = nullptr;            // small pinned host buffer for scalar readback
+};
+
+namespace tenrso {
+
+enum BufSlot : int {
+  SLOT_PARTIAL = 0,
+  SLOT_M64,
+  SLOT_GRAMS,
+  SLOT_GRAM_PART,
+  SLOT_SOLVE,
+  SLOT_SCALARS,
+  SLOT_XBF16,
+  SLOT_XBF16_T,
+  SLOT_STAGE_X,
+  SLOT_STAGE_F,
+  SLOT_STAGE_OUT,
+  SLOT_TMP,
+  SLOT_TMP2,
+  SLOT_COUNT
+};
+
+int ctx_buffer(tenrso_ctx* ctx, int slot, size_t bytes, void** out);
+
+inline int set_device(tenrso_ctx* ctx) {
+  TENRSO_HIP_TRY(hipSetDevice(ctx->device));
+  return TENRSO_OK;
+}
+
+}  // namespace tenrso

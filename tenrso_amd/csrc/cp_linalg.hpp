@@ -1,0 +1,53 @@
+// cp_linalg.hpp -- FP64 normal-equation kernels of the ALS step.
+#pragma once
+#include "common.hpp"
+
+namespace tenrso {
+
+constexpr int SOLVE_RMAX = 64;  // rank bound of the on-chip (LDS) normal-equation solve
+
+// Regularisation applied to the Gram-Hadamard matrix (cp/core.rs:330-345).
+struct GramReg {
+  double l2 = 0.0;        // ridge on the diagonal
+  int tikhonov_order = 0; // > 0: lambda * D^T D (cp/helpers.rs:630-702)
+  double tikhonov_lambda = 0.0;
+};
+
+// Epilogue applied to the solved rows (cp/core.rs:351-365).
+struct RowEpilogue {
+  double l1 = 0.0;        // soft threshold
+  int nonneg = 0;         // max(0, x)
+};
+
+// Device scalars block used by the ALS loop (doubles).
+enum Scalars : int {
+  SC_NORM_X = 0,   // ||X||^2
+  SC_FIT = 1,      // fit of the last evaluation
+  SC_IP = 2,       // <X, Xhat>
+  SC_RECON = 3,    // ||Xhat||^2
+  SC_STATUS = 4,   // solve path: 0 Cholesky-class, 1 ridge retry, 2 failure
+  SC_MIN_X = 5,    // min(X)
+  SC_COUNT = 8
+};
+
+// ||X||^2 (FP64) and min(X) into scal[SC_NORM_X], scal[SC_MIN_X].  tmp: >= 2*1024 doubles.
+int tensor_norm_min(tenrso_ctx* ctx, const float* X, i64 n, double* tmp, double* scal);
+// G = A^T A (FP64) for an fp32 rows x R factor.
+int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G);
+// Winv = (hadamard_{k != skip} G_k + reg)^{-1}; G: [nd][R][R].  Writes the path taken
+// into scal[SC_STATUS] (max with the previous value).
+int normal_eq_inverse(tenrso_ctx* ctx, const double* G, int nd, int skip, i64 R, const GramReg& reg,
+                      double* Winv, double* scal);
+// A[i,:] = epi(M[i,:] Winv) for an fp64 M; ip_part[blk] = sum_i M[i,:].A[i,:] (FP64).
+int solve_rows(tenrso_ctx* ctx, const double* M, i64 rows, i64 R, const double* Winv, const RowEpilogue& epi,
+               float* A, double* ip_part, int* nblk_out);
+// ip_part[blk] = sum_{i in blk} M[i,:] . A[i,:]
+int inner_rows(tenrso_ctx* ctx, const double* M, const float* A, i64 rows, i64 R, double* ip_part, int* nblk_out);
+// fit from the Grams, the ip partials and ||X||^2 (cp/helpers.rs:513-555).
+int fit_from_grams(tenrso_ctx* ctx, const double* G, int nd, i64 R, const double* ip_part, int nblk, double* scal);
+// x = max(x, 0) on an fp32 array.
+int clamp_nonneg(tenrso_ctx* ctx, float* A, i64 n);
+// fill an fp64 array
+int fill_f64(tenrso_ctx* ctx, double* p, i64 n, double v);
+
+}  // namespace tenrso

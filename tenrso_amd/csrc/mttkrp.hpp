@@ -1,0 +1,40 @@
+// mttkrp.hpp -- host-side interface of the dense MTTKRP kernels.
+#pragma once
+#include "common.hpp"
+
+namespace tenrso {
+
+constexpr int MAXD = 8;
+
+// Output form of a dense MTTKRP launch.
+enum class MttkrpOut {
+  F32,      // reduce split partials into an fp32 I x R result
+  F64,      // reduce split partials into an fp64 I x R result
+  PARTIAL   // leave [splits, I, R] fp32 partials (caller reduces, e.g. the ALS solve)
+};
+
+struct MttkrpResult {
+  const float* partial = nullptr;  // [splits, I, R]
+  int splits = 0;
+};
+
+// X: fp32 tensor (prec FP32) or bf16 tensor (prec BF16, uint16 storage).  For BF16
+// and mode == ndim-1 the caller may pass X_t, the tensor with the last mode moved
+// to the front (see make_bf16_copies); if X_t is null the launcher builds it.
+int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int nd, const i64* shape,
+                 const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res);
+
+// Sum split partials [splits, n] in FP64 into out (fp32 or fp64).
+int reduce_partials(tenrso_ctx* ctx, const float* partial, int splits, i64 n, float* out32, double* out64);
+
+// fp32 -> bf16 (RNE) copy; when `last_to_front` the output is the tensor with its
+// last mode moved to the front ([I_{N-1}, I_0, ..., I_{N-2}]).
+int convert_bf16(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, bool last_to_front, uint16_t* dst);
+
+// bf16 [.., I_{N-1}] -> [I_{N-1}, ..] (last mode moved to the front), plain copy.
+int transpose_last_to_front_u16(tenrso_ctx* ctx, const uint16_t* src, int nd, const i64* shape, uint16_t* dst);
+
+// Forward-order Khatri-Rao of `n` factors (rows = prod sizes) into dst (fp32).
+int khatri_rao_many(tenrso_ctx* ctx, int n, const float* const* F, const i64* sizes, i64 R, float* dst);
+
+}  // namespace tenrso

@@ -1,0 +1,214 @@
+"""tenrso-kernels mirror: MTTKRP, Khatri-Rao and CP reconstruction on gfx950.
+
+Same names, argument meaning and error behaviour as the reference functions
+(tenrso-kernels/src/mttkrp.rs, khatri_rao.rs, outer.rs, utils.rs), for T = f32:
+
+* numpy inputs take the host path (``*_host`` C entry points: copy in, compute on
+  the GPU, copy out) and return numpy arrays;
+* torch CUDA tensors take the device path on torch's current stream and return
+  torch tensors (no host round trip).
+
+Validation failures raise ``ValueError`` carrying the reference's message;
+device failures raise ``TenrsoError``.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+__all__ = [
+    "mttkrp", "mttkrp_blocked", "mttkrp_blocked_parallel", "mttkrp_fused", "mttkrp_fused_parallel",
+    "mttkrp_all_modes", "mttkrp_bf16", "to_bf16", "khatri_rao", "khatri_rao_parallel", "khatri_rao_blocked",
+    "khatri_rao_blocked_parallel", "cp_reconstruct",
+]
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _i64s(vals):
+    return (C.c_int64 * len(vals))(*[int(v) for v in vals])
+
+
+def _check_mttkrp(shape, factors, mode):
+    """mttkrp.rs:85-122 validation (same order, same messages) via tenrso_mttkrp_check."""
+    nf = len(factors)
+    if nf == 0:
+        raise ValueError(f"Number of factor matrices (0) must match tensor rank ({len(shape)})")
+    rows = [int(f.shape[0]) for f in factors]
+    cols = [int(f.shape[1]) if len(f.shape) == 2 else -1 for f in factors]
+    rc = lib().tenrso_mttkrp_check(len(shape), _i64s(shape), nf, _i64s(rows), _i64s(cols), int(mode))
+    if rc != _lib.OK:
+        raise ValueError(_lib.last_error())
+    return cols[0]
+
+
+def _np32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def mttkrp(tensor, factors, mode: int):
+    """tenrso_kernels::mttkrp (mttkrp.rs:74-146): V = X_(mode) . KR(factors except mode).
+
+    Returns an (I_mode, R) array/tensor.
+    """
+    shape = tuple(int(s) for s in tensor.shape)
+    R = _check_mttkrp(shape, factors, mode)
+    if _is_torch(tensor):
+        import torch
+        X = tensor.contiguous()
+        if X.dtype != torch.float32 or not X.is_cuda:
+            raise TypeError("device mttkrp expects a float32 CUDA tensor")
+        F = [f.contiguous() for f in factors]
+        out = torch.empty((shape[mode], R), dtype=torch.float32, device=X.device)
+        ctx = _lib.torch_context(X)
+        ptrs = (C.c_void_p * len(F))(*[f.data_ptr() for f in F])
+        check(lib().tenrso_mttkrp_f32(ctx.ptr, X.data_ptr(), len(shape), _i64s(shape), ptrs, R, mode,
+                                      out.data_ptr()))
+        return out
+    X = _np32(tensor)
+    F = [_np32(f) for f in factors]
+    out = np.empty((shape[mode], R), dtype=np.float32)
+    ctx = _lib.context_for(0)
+    ptrs = (C.c_void_p * len(F))(*[f.ctypes.data for f in F])
+    check(lib().tenrso_mttkrp_f32_host(ctx.ptr, X.ctypes.data, len(shape), _i64s(shape), ptrs, R, mode,
+                                       out.ctypes.data))
+    return out
+
+
+def _tile_check(tile_size):
+    if int(tile_size) == 0:
+        raise ValueError("Tile size must be greater than 0")
+
+
+def mttkrp_blocked(tensor, factors, mode: int, tile_size: int):
+    """mttkrp_blocked (mttkrp.rs:395-498): same contract as mttkrp; tiling is internal."""
+    _tile_check(tile_size)
+    return mttkrp(tensor, factors, mode)
+
+
+def mttkrp_blocked_parallel(tensor, factors, mode: int, tile_size: int):
+    """mttkrp_blocked_parallel (mttkrp.rs:528-609)."""
+    _tile_check(tile_size)
+    return mttkrp(tensor, factors, mode)
+
+
+def mttkrp_fused(tensor, factors, mode: int):
+    """mttkrp_fused (mttkrp.rs:783-878): KR formed on the fly (as every path here)."""
+    return mttkrp(tensor, factors, mode)
+
+
+def mttkrp_fused_parallel(tensor, factors, mode: int):
+    """mttkrp_fused_parallel (mttkrp.rs:903-992)."""
+    return mttkrp(tensor, factors, mode)
+
+
+def mttkrp_all_modes(tensor, factors):
+    """mttkrp_all_modes (utils.rs:282-311)."""
+    return [mttkrp(tensor, factors, m) for m in range(len(tensor.shape))]
+
+
+def to_bf16(tensor):
+    """Round a float32 CUDA tensor to bfloat16 (RNE) with the library's conversion kernel."""
+    import torch
+    X = tensor.contiguous()
+    out = torch.empty(X.shape, dtype=torch.bfloat16, device=X.device)
+    ctx = _lib.torch_context(X)
+    check(lib().tenrso_convert_f32_to_bf16(ctx.ptr, X.data_ptr(), X.numel(), out.data_ptr()))
+    return out
+
+
+def mttkrp_bf16(tensor_bf16, factors, mode: int):
+    """BF16-input / FP32-accumulate MFMA MTTKRP (no reference counterpart).
+
+    tensor_bf16: torch.bfloat16 CUDA tensor; factors: float32 CUDA tensors.
+    """
+    import torch
+    shape = tuple(int(s) for s in tensor_bf16.shape)
+    R = _check_mttkrp(shape, factors, mode)
+    X = tensor_bf16.contiguous()
+    if X.dtype != torch.bfloat16:
+        raise TypeError("mttkrp_bf16 expects a bfloat16 tensor (see to_bf16)")
+    F = [f.contiguous().float() for f in factors]
+    out = torch.empty((shape[mode], R), dtype=torch.float32, device=X.device)
+    ctx = _lib.torch_context(X)
+    ptrs = (C.c_void_p * len(F))(*[f.data_ptr() for f in F])
+    check(lib().tenrso_mttkrp_bf16(ctx.ptr, X.data_ptr(), len(shape), _i64s(shape), ptrs, R, mode,
+                                   out.data_ptr()))
+    return out
+
+
+def khatri_rao(a, b):
+    """khatri_rao (khatri_rao.rs:56-89): C[ia*J + ib, k] = A[ia,k] * B[ib,k].
+
+    The reference panics on a column mismatch; here that raises ValueError with
+    the same message.
+    """
+    if a.shape[1] != b.shape[1]:
+        raise ValueError(f"Number of columns must match: A has {a.shape[1]} columns, B has {b.shape[1]} columns")
+    I, K = int(a.shape[0]), int(a.shape[1])
+    J = int(b.shape[0])
+    if _is_torch(a):
+        import torch
+        A, B = a.contiguous().float(), b.contiguous().float()
+        out = torch.empty((I * J, K), dtype=torch.float32, device=A.device)
+        ctx = _lib.torch_context(A)
+        check(lib().tenrso_khatri_rao_f32(ctx.ptr, A.data_ptr(), I, B.data_ptr(), J, K, out.data_ptr()))
+        return out
+    A, B = _np32(a), _np32(b)
+    out = np.empty((I * J, K), dtype=np.float32)
+    if out.size:
+        ctx = _lib.context_for(0)
+        check(lib().tenrso_khatri_rao_f32_host(ctx.ptr, A.ctypes.data, I, B.ctypes.data, J, K, out.ctypes.data))
+    return out
+
+
+def khatri_rao_parallel(a, b):
+    """khatri_rao_parallel (khatri_rao.rs:117-157): bit-identical to khatri_rao."""
+    return khatri_rao(a, b)
+
+
+def khatri_rao_blocked(a, b, block_size: int):
+    """khatri_rao_blocked (khatri_rao.rs:193-255): panics on block_size 0."""
+    if a.shape[1] != b.shape[1]:
+        raise ValueError(f"Number of columns must match: A has {a.shape[1]} columns, B has {b.shape[1]} columns")
+    if int(block_size) == 0:
+        raise ValueError(f"Block size must be greater than 0, got {block_size}")
+    return khatri_rao(a, b)
+
+
+def khatri_rao_blocked_parallel(a, b, block_size: int):
+    """khatri_rao_blocked_parallel (khatri_rao.rs:279-357)."""
+    return khatri_rao_blocked(a, b, block_size)
+
+
+def cp_reconstruct(factors, weights=None):
+    """cp_reconstruct (outer.rs:223-274): sum_r w_r a_r^(0) o ... o a_r^(N-1)."""
+    if len(factors) == 0:
+        raise ValueError("Need at least one factor matrix for CP reconstruction")
+    R = int(factors[0].shape[1])
+    for i, f in enumerate(factors):
+        if int(f.shape[1]) != R:
+            raise ValueError(f"Factor {i} has {f.shape[1]} columns, expected {R}")
+    if weights is not None and len(weights) != R:
+        raise ValueError(f"Weights length {len(weights)} must match rank {R}")
+    shape = [int(f.shape[0]) for f in factors]
+    if _is_torch(factors[0]):
+        import torch
+        F = [f.contiguous().float() for f in factors]
+        w = None if weights is None else weights.contiguous().float()
+        out = torch.empty(shape, dtype=torch.float32, device=F[0].device)
+        ctx = _lib.torch_context(F[0])
+        ptrs = (C.c_void_p * len(F))(*[f.data_ptr() for f in F])
+        check(lib().tenrso_cp_reconstruct_f32(ctx.ptr, len(F), _i64s(shape), ptrs, R,
+                                              None if w is None else w.data_ptr(), out.data_ptr()))
+        return out
+    import torch
+    F = [torch.from_numpy(_np32(f)).cuda() for f in factors]
+    w = None if weights is None else torch.from_numpy(_np32(weights)).cuda()
+    return cp_reconstruct(F, w).cpu().numpy()

@@ -1,0 +1,189 @@
+"""GPU parity of the dense path: HIP library (through the C ABI) vs the FP64 oracle.
+
+Bars (written in each test): FP32 MTTKRP relative Frobenius error <= 1e-5 vs FP64;
+Khatri-Rao bit-exact vs the f32 restatement; CP-ALS fit within 1e-4 of the FP64
+oracle run from the same initial factors.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import oracle  # noqa: E402
+from tenrso_amd import decomp, kernels, synth  # noqa: E402
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def f64(fs):
+    return [np.asarray(f, dtype=np.float64) for f in fs]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    print(f"[gpu] {torch.cuda.get_device_name(0)}", flush=True)
+
+
+# shapes: fast MFMA path (fastest other mode % 32 == 0), ragged rows, fallback path,
+# 2-way, 4-way and 5-way tensors
+SHAPES = [
+    ((2, 3, 4), 2), ((4, 5, 6), 3), ((40, 64, 32), 8), ((17, 32, 64), 5), ((64, 96, 32), 16), ((33, 50, 70), 7),
+    ((128, 64), 20), ((24, 32, 16, 32), 6), ((6, 5, 4, 3, 32), 3), ((100, 64, 64), 64), ((20, 32, 32), 70),
+]
+
+
+@pytest.mark.parametrize("shape,rank", SHAPES)
+def test_mttkrp_f32_all_modes_vs_fp64(shape, rank):
+    print(f"[case] mttkrp f32 shape={shape} rank={rank}", flush=True)
+    rng = np.random.default_rng(hash(shape) % 2**32)
+    X = rng.standard_normal(shape).astype(np.float32)
+    F = [rng.standard_normal((n, rank)).astype(np.float32) for n in shape]
+    Xd, Fd = dev(X), [dev(f) for f in F]
+    for mode in range(len(shape)):
+        got = kernels.mttkrp(Xd, Fd, mode).cpu().numpy()
+        ref = oracle.mttkrp_fast(X.astype(np.float64), f64(F), mode)
+        assert got.shape == ref.shape
+        assert rel(got, ref) <= 1e-5, (mode, rel(got, ref))
+        # host path (numpy in, numpy out) gives the same values
+        if np.prod(shape) <= 100_000:
+            host = kernels.mttkrp(X, F, mode)
+            assert rel(host, ref) <= 1e-5
+
+
+def test_mttkrp_reference_kats_on_gpu():
+    # mttkrp.rs:1000-1043 and property_tests.rs:208-229 evaluated by the HIP path
+    X = np.arange(1, 13, dtype=np.float32).reshape(2, 3, 2)
+    u1 = np.array([[1.0, 0.0], [2.0, 1.0]], np.float32)
+    u2 = np.array([[1.0, 0.0], [0.5, 1.0], [0.0, 1.0]], np.float32)
+    u3 = np.array([[1.0, 0.0], [0.0, 1.0]], np.float32)
+    manual = np.zeros((3, 2))
+    for i1 in range(3):
+        for r in range(2):
+            manual[i1, r] = sum(X[i0, i1, i2] * u1[i0, r] * u3[i2, r] for i0 in range(2) for i2 in range(2))
+    assert np.abs(kernels.mttkrp(dev(X), [dev(u1), dev(u2), dev(u3)], 1).cpu().numpy() - manual).max() < 1e-5
+    for size, rank in [(3, 2), (7, 4), (32, 3), (64, 16)]:
+        ones = np.ones((size,) * 3, np.float32)
+        F = [np.ones((size, rank), np.float32)] * 3
+        for mode in range(3):
+            got = kernels.mttkrp(dev(ones), [dev(f) for f in F], mode).cpu().numpy()
+            assert np.abs(got - size * size).max() < 1e-3
+
+
+def test_mttkrp_c2_shape_fp32():
+    print("[case] 256^3 rank-64 model tensor, all modes", flush=True)
+    X, F, w, init = synth.dense_256_r64()
+    Xd = dev(X)
+    Fd = [dev(f) for f in init]
+    X64 = X.astype(np.float64)
+    for mode in range(3):
+        got = kernels.mttkrp(Xd, Fd, mode).cpu().numpy()
+        ref = oracle.mttkrp_fast(X64, f64(init), mode)
+        e = rel(got, ref)
+        print(f"  mode {mode}: rel frob err {e:.3e}", flush=True)
+        assert e <= 1e-5
+
+
+def test_mttkrp_bf16_error_bound():
+    print("[case] bf16 mttkrp 128x128x64 rank 32", flush=True)
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((128, 128, 64)).astype(np.float32)
+    F = [rng.standard_normal((n, 32)).astype(np.float32) for n in X.shape]
+    Xd = dev(X)
+    Xb = kernels.to_bf16(Xd)
+    # conversion is RNE, identical to torch's
+    assert torch.equal(Xb.view(torch.int16), Xd.to(torch.bfloat16).view(torch.int16))
+    Fd = [dev(f) for f in F]
+    for mode in range(3):
+        got = kernels.mttkrp_bf16(Xb, Fd, mode).cpu().numpy()
+        ref = oracle.mttkrp_fast(X.astype(np.float64), f64(F), mode)
+        # exact MTTKRP of the bf16-rounded tensor and bf16-rounded KR rows, FP64
+        e = rel(got, ref)
+        print(f"  mode {mode}: rel frob err vs fp64 {e:.3e}", flush=True)
+        assert e < 5e-3
+
+
+def test_khatri_rao_bit_exact():
+    rng = np.random.default_rng(2)
+    for (I, J, K) in [(2, 2, 2), (3, 2, 2), (37, 19, 5), (500, 64, 32), (128, 128, 64)]:
+        A = rng.standard_normal((I, K)).astype(np.float32)
+        B = rng.standard_normal((J, K)).astype(np.float32)
+        ref = oracle.khatri_rao(A, B, dtype=np.float32)
+        assert np.array_equal(kernels.khatri_rao(dev(A), dev(B)).cpu().numpy(), ref)
+        assert np.array_equal(kernels.khatri_rao(A, B), ref)
+    # reference KAT khatri_rao.rs:41-55
+    c = kernels.khatri_rao(np.array([[1.0, 2.0], [3.0, 4.0]]), np.array([[5.0, 6.0], [7.0, 8.0]]))
+    assert c[:, 0].tolist() == [5, 7, 15, 21] and c[:, 1].tolist() == [12, 16, 24, 32]
+
+
+def test_cp_reconstruct_vs_oracle():
+    rng = np.random.default_rng(3)
+    for shape, R in [((3, 4, 5), 3), ((32, 16, 8, 4), 5), ((64, 64, 64), 16)]:
+        F = [rng.standard_normal((n, R)).astype(np.float32) for n in shape]
+        w = rng.random(R).astype(np.float32)
+        got = kernels.cp_reconstruct([dev(f) for f in F], dev(w)).cpu().numpy()
+        ref = oracle.cp_reconstruct(f64(F), w.astype(np.float64))
+        assert rel(got, ref) < 1e-5
+
+
+def test_cp_als_small_vs_oracle():
+    print("[case] cp_als 64^3 rank 8 (20 iters, tol 1e-4)", flush=True)
+    X, _, init = synth.dense_64_r8()
+    cp = decomp.cp_als_with_init(dev(X), [dev(f) for f in init], 20, 1e-4)
+    ref_F, info = oracle.cp_als(X, init, 20, 1e-4)
+    print(f"  gpu fit {cp.fit:.8f} iters {cp.iters} {cp.convergence.reason}; "
+          f"oracle fit {info['fit']:.8f} iters {info['iters']} {info['reason']}", flush=True)
+    assert abs(cp.fit - info["fit"]) <= 1e-4
+    assert cp.iters == info["iters"]
+    assert cp.convergence.reason.name == info["reason"]
+    hist = np.array(cp.convergence.fit_history)
+    assert np.abs(hist - info["fit_history"]).max() <= 1e-4
+    for g, r in zip(cp.factors, ref_F):
+        assert rel(g.cpu().numpy(), r) < 1e-3
+
+
+def test_cp_als_c2_vs_oracle():
+    print("[case] cp_als 256^3 rank 64, 10 iters", flush=True)
+    X, _, _, init = synth.dense_256_r64()
+    cp = decomp.cp_als_with_init(dev(X), [dev(f) for f in init], 10, 0.0)
+    ref_F, info = oracle.cp_als(X, init, 10, 0.0)
+    errs = [rel(g.cpu().numpy(), r) for g, r in zip(cp.factors, ref_F)]
+    print(f"  gpu fit {cp.fit:.8f} oracle {info['fit']:.8f} |dfit| {abs(cp.fit - info['fit']):.2e} "
+          f"factor rel errs {['%.2e' % e for e in errs]} gpu {cp.seconds:.4f}s oracle {info['seconds']:.1f}s",
+          flush=True)
+    assert abs(cp.fit - info["fit"]) <= 1e-4
+    assert max(errs) < 1e-3
+
+
+def test_cp_als_host_path_and_errors():
+    X, _, init = synth.dense_64_r8()
+    cp = decomp.cp_als_with_init(X, init, 3, 0.0)
+    _, info = oracle.cp_als(X, init, 3, 0.0)
+    assert abs(cp.fit - info["fit"]) <= 1e-4 and isinstance(cp.factors[0], np.ndarray)
+    with pytest.raises(decomp.InvalidRank, match="Invalid rank: 0"):
+        decomp.cp_als(X, 0, 5, 1e-4)
+    with pytest.raises(decomp.InvalidRank):
+        decomp.cp_als(X, 65, 5, 1e-4)
+    with pytest.raises(decomp.InvalidTolerance):
+        decomp.cp_als(X, 4, 5, 1.5)
+    cp0 = decomp.cp_als_with_init(X, init, 5, 1e-4, time_limit=0.0)
+    assert cp0.iters == 0 and cp0.convergence.reason == decomp.ConvergenceReason.TimeLimit
+
+
+def test_cp_als_bf16_fit_close():
+    print("[case] cp_als bf16 256^3 rank 64, 10 iters", flush=True)
+    X, _, _, init = synth.dense_256_r64()
+    cp = decomp.cp_als_with_init(dev(X), [dev(f) for f in init], 10, 0.0, precision="bf16")
+    _, info = oracle.cp_als(X, init, 10, 0.0)
+    print(f"  bf16 fit {cp.fit:.8f} oracle {info['fit']:.8f} |dfit| {abs(cp.fit - info['fit']):.2e}", flush=True)
+    assert abs(cp.fit - info["fit"]) <= 1e-4
