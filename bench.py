@@ -1,0 +1,182 @@
+"""Benchmark: CP-ALS seconds per iteration and MTTKRP GB/s on one MI355X.
+
+Workload (dense_256_r64): 256^3 float32 tensor (64 MiB) from a rank-64 CP model
+(factor columns N(0,1) normalised, weights U(0.5, 2), noise sigma 1e-3), rank-64
+CP-ALS from seeded N(0,1) initial factors.  A "step" is one ALS iteration (three
+mode updates + fit).  The headline value is the FP32 drop-in path; the
+BF16-input / FP32-accumulate MFMA path is timed on the same tensor and reported
+beside it, with each fit checked against the FP64 oracle run from the same
+initial factors (in the cpu_baseline leg).
+
+Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "CP-ALS s/iter + MTTKRP GB/s, 256³ r=64: BF16-MFMA vs FP32, fit Δ≤1e-4"
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def mttkrp_bytes(shape, R, mode, x_bytes):
+    """Algorithmic bytes of one dense MTTKRP: X once, the other factors, the output (fp32)."""
+    n = 1
+    for s in shape:
+        n *= s
+    fac = sum(shape[k] * R * 4 for k in range(len(shape)) if k != mode)
+    return n * x_bytes + fac + shape[mode] * R * 4
+
+
+def mttkrp_flops(shape, R):
+    n = 1
+    for s in shape:
+        n *= s
+    return 2 * n * R
+
+
+def profile_kernel(ctx, prec, X, shape, F, R, mode, reps):
+    from tenrso_amd import _lib
+    ms = C.c_double()
+    ptrs = (C.c_void_p * len(F))(*[f.data_ptr() for f in F])
+    shp = (C.c_int64 * len(shape))(*shape)
+    _lib.check(_lib.lib().tenrso_mttkrp_profile(ctx.ptr, prec, X.data_ptr(), len(shape), shp, ptrs, R, mode, reps,
+                                                C.byref(ms)))
+    return ms.value
+
+
+def cpu_baseline(X, init, gpu_fits, max_iters):
+    """CPU leg: the faithful f32 restatement of the reference cp_als (single thread),
+    timed on a bounded sample of one ALS iteration of the same workload, plus the
+    FP64 oracle run of the full iteration count as the fit checker."""
+    import numpy as np
+    import oracle
+    t0 = time.perf_counter()
+    _, info32 = oracle.cp_als(X, init, 1, 0.0, dtype=np.float32)
+    t_iter = time.perf_counter() - t0
+    _, info64 = oracle.cp_als(X, init, max_iters, 0.0)
+    return {
+        "value": t_iter,
+        "unit": "s/iter",
+        "cores": 1,
+        "kind": "port",
+        "sample": "1 iteration of the faithful f32 restatement of cp_als (3 mode MTTKRPs with unfolding copy and "
+                  "materialised Khatri-Rao + per-row solves + the fit's mode-0 MTTKRP) on the same 256^3 r64 tensor",
+        "cpu": platform.processor() or platform.machine(),
+        "host_threads_available": os.cpu_count(),
+        "oracle_fp64_fit": info64["fit"],
+        "fit_delta": {k: abs(v - info64["fit"]) for k, v in gpu_fits.items()},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--reps", type=int, default=20, help="launches per MTTKRP kernel timing")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is described in "
+              "DESIGN.md and not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from tenrso_amd import _lib, decomp, kernels, synth
+
+    dev = torch.device("cuda:0")
+    X, _, _, init = synth.dense_256_r64()
+    shape = tuple(X.shape)
+    R = init[0].shape[1]
+    Xd = torch.from_numpy(X).to(dev)
+    Fd = [torch.from_numpy(f).to(dev) for f in init]
+    K, W = args.steps, args.warmup
+
+    def timed_als(precision):
+        decomp.cp_als_with_init(Xd, Fd, max(W, 1), 0.0, precision=precision)  # warmup (W iterations)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        cp = decomp.cp_als_with_init(Xd, Fd, K, 0.0, precision=precision)
+        e1.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return cp, e0.elapsed_time(e1) / K, wall
+
+    cp32, ms32, wall32 = timed_als("fp32")
+    cp16, ms16, wall16 = timed_als("bf16")
+
+    # per-mode MTTKRP kernels, timed with HIP events on the launch stream
+    ctx = _lib.torch_context(Xd)
+    Xb = kernels.to_bf16(Xd)
+    torch.cuda.synchronize()
+    modes = {}
+    for mode in range(len(shape)):
+        t32 = profile_kernel(ctx, _lib.PREC_FP32, Xd, shape, Fd, R, mode, args.reps)
+        t16 = profile_kernel(ctx, _lib.PREC_BF16, Xb, shape, Fd, R, mode, args.reps)
+        b32, b16 = mttkrp_bytes(shape, R, mode, 4), mttkrp_bytes(shape, R, mode, 2)
+        fl = mttkrp_flops(shape, R)
+        modes[str(mode)] = {
+            "fp32_us": t32 * 1e3, "fp32_GBs": b32 / (t32 * 1e-3) / 1e9, "fp32_TFLOPs": fl / (t32 * 1e-3) / 1e12,
+            "bf16_us": t16 * 1e3, "bf16_GBs": b16 / (t16 * 1e-3) / 1e9, "bf16_TFLOPs": fl / (t16 * 1e-3) / 1e12,
+        }
+    # dominant kernel: the fp32 MFMA instance of modes 0 and 1 (q-contiguous tiles)
+    t_dom = (modes["0"]["fp32_us"] + modes["1"]["fp32_us"]) / 2.0
+    ach = mttkrp_flops(shape, R) / (t_dom * 1e-6) / 1e12
+    roofline = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
+                "kernel": "mttkrp_f32_mfma<4,4,2,false,128> (modes 0,1)", "avg_us": t_dom,
+                "algorithmic_flops": mttkrp_flops(shape, R)}
+    t_b16 = (modes["0"]["bf16_us"] + modes["1"]["bf16_us"]) / 2.0
+    b_alg = mttkrp_bytes(shape, R, 0, 2)
+    roof_bf16 = {"bound": "hbm", "achieved": b_alg / (t_b16 * 1e-6) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                 "frac": b_alg / (t_b16 * 1e-6) / 1e9 / PEAK_HBM_GBS, "kernel": "mttkrp_bf16_mfma<4,4,2,2,128> (modes 0,1)",
+                 "avg_us": t_b16, "algorithmic_bytes": b_alg}
+
+    gpu_fits = {"fp32": cp32.fit, "bf16": cp16.fit}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(X, init, gpu_fits, K)
+
+    out = {
+        "metric": METRIC,
+        "value": ms32 / 1e3,
+        "unit": "s/iter",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms32,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded CP model tensor, numpy PCG64)",
+        "config": {"workload": "dense_256_r64: 256x256x256 f32, rank-64 CP model + noise 1e-3, cp_als rank 64, "
+                               "seeded N(0,1) init, tol 0 (fixed iteration count)",
+                   "shape": list(shape), "rank": R, "iters_timed": K},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "bf16": {"s_per_iter": ms16 / 1e3, "ms_per_step": ms16, "speedup_vs_fp32": ms32 / ms16,
+                 "roofline": roof_bf16, "fit": cp16.fit},
+        "fp32": {"s_per_iter": ms32 / 1e3, "fit": cp32.fit, "wall_s": wall32},
+        "mttkrp_per_mode": modes,
+        "speedup_vs_cpu_port": (cpu["value"] / (ms32 / 1e3)) if cpu else None,
+        "gpu": torch.cuda.get_device_name(0),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

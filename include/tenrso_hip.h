@@ -108,6 +108,13 @@ int tenrso_mttkrp_bf16(tenrso_ctx* ctx, const uint16_t* X_bf16, int ndim, const 
                        const float* const* factors, int64_t rank, int mode, float* out);
 int tenrso_convert_f32_to_bf16(tenrso_ctx* ctx, const float* src, int64_t n, uint16_t* dst);
 
+/* Profiling aid (no reference counterpart): launches only the main MFMA (or
+ * fallback) MTTKRP kernel `reps` times back to back between two HIP events on
+ * the context's stream and returns the average device time per launch in *ms.
+ * precision selects the fp32 (X: float*) or bf16 (X: uint16_t*) kernel. */
+int tenrso_mttkrp_profile(tenrso_ctx* ctx, int precision, const void* X, int ndim, const int64_t* shape,
+                          const float* const* factors, int64_t rank, int mode, int reps, double* ms);
+
 /* mttkrp_all_modes (tenrso-kernels/src/utils.rs:282-311): outs[k] for every mode. */
 int tenrso_mttkrp_all_modes_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
                                 const float* const* factors, int64_t rank, float* const* outs);

@@ -171,6 +171,38 @@ int tenrso_mttkrp_bf16(tenrso_ctx* ctx, const uint16_t* X_bf16, int ndim, const 
   return timed_end(ctx);
 }
 
+int tenrso_mttkrp_profile(tenrso_ctx* ctx, int precision, const void* X, int ndim, const int64_t* shape,
+                          const float* const* factors, int64_t rank, int mode, int reps, double* ms) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  TENRSO_TRY(check_mttkrp_basic(ndim, shape, rank, mode));
+  if (reps <= 0 || !ms) {
+    set_error("reps must be positive and ms non-null");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  if (rank == 0 || shape[mode] == 0) {
+    *ms = 0.0;
+    return TENRSO_OK;
+  }
+  TENRSO_TRY(set_device(ctx));
+  const bool bf16 = precision == TENRSO_PREC_BF16;
+  const void* Xt = nullptr;
+  if (bf16 && mode == ndim - 1) {
+    const i64 total = prod(shape, ndim);
+    void* pt = nullptr;
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * 2 + 16, &pt));
+    TENRSO_TRY(transpose_last_to_front_u16(ctx, (const uint16_t*)X, ndim, shape, (uint16_t*)pt));
+    Xt = pt;
+  }
+  ctx->profile_reps = reps;
+  int rc = mttkrp_dense(ctx, bf16 ? TENRSO_PREC_BF16 : TENRSO_PREC_FP32, X, Xt, ndim, shape, factors, rank, mode,
+                        MttkrpOut::PARTIAL, nullptr, nullptr);
+  ctx->profile_reps = 0;
+  if (rc != TENRSO_OK) return rc;
+  *ms = ctx->last_kernel_ms;
+  return TENRSO_OK;
+}
+
 int tenrso_mttkrp_all_modes_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
                                 const float* const* factors, int64_t rank, float* const* outs) {
   for (int m = 0; m < ndim; ++m) TENRSO_TRY(tenrso_mttkrp_f32(ctx, X, ndim, shape, factors, rank, m, outs[m]));

@@ -1,4 +1,6 @@
 // context.hip -- context lifetime, device buffer pool and error strings.
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #include "common.hpp"
@@ -91,6 +93,9 @@ int tenrso_ctx_create(int device, int borrow_stream, void* stream, tenrso_ctx** 
     c->own_stream = true;
   }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_g, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_w, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
     set_error("context setup failed");
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -98,6 +103,8 @@ int tenrso_ctx_create(int device, int borrow_stream, void* stream, tenrso_ctx** 
     return TENRSO_ERR_DEVICE;
   }
   c->bufs.resize(SLOT_COUNT);
+  if (const char* e = std::getenv("TENRSO_WG_PER_CU_F32")) c->wg_per_cu_f32 = std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("TENRSO_WG_PER_CU_BF16")) c->wg_per_cu_bf16 = std::max(1, std::atoi(e));
   *out = c;
   return TENRSO_OK;
 }
@@ -110,6 +117,12 @@ int tenrso_ctx_destroy(tenrso_ctx* c) {
     if (b.ptr) (void)hipFree(b.ptr);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev_g) (void)hipEventDestroy(c->ev_g);
+  if (c->ev_w) (void)hipEventDestroy(c->ev_w);
+  if (c->aux) {
+    (void)hipStreamSynchronize(c->aux);
+    (void)hipStreamDestroy(c->aux);
+  }
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;

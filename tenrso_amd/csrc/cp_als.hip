@@ -76,9 +76,9 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
   void *p_scal, *p_G, *p_Winv, *p_M, *p_ip, *p_tmp;
   TENRSO_TRY(ctx_buffer(ctx, SLOT_SCALARS, SC_COUNT * sizeof(double), &p_scal));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_GRAMS, (size_t)(nd + 1) * R * R * sizeof(double), &p_G));
-  TENRSO_TRY(ctx_buffer(ctx, SLOT_SOLVE, (size_t)R * R * sizeof(double), &p_Winv));
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_SOLVE, (size_t)SOLVE_RMAX * SOLVE_RMAX * sizeof(double), &p_Winv));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_M64, (size_t)maxI * R * sizeof(double), &p_M));
-  TENRSO_TRY(ctx_buffer(ctx, SLOT_TMP2, (size_t)std::max<i64>(4096, (maxI + 15) / 16 + 16) * sizeof(double), &p_tmp));
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_TMP2, (size_t)std::max<i64>(4096, maxI + 16) * sizeof(double), &p_tmp));
   p_ip = p_tmp;
   double* scal = (double*)p_scal;
   double* G = (double*)p_G;
@@ -140,10 +140,17 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
     int nblk = 0;
     for (int mode = 0; mode < nd; ++mode) {
       const i64 I = shape[mode];
-      TENRSO_TRY(normal_eq_inverse(ctx, G, nd, mode, R, reg, Winv, scal));
+      // Gamma_n^{-1} only needs the Grams, so it runs on the side stream while
+      // the MTTKRP of the same mode runs on the main stream.
+      TENRSO_HIP_TRY(hipEventRecord(ctx->ev_g, ctx->stream));
+      TENRSO_HIP_TRY(hipStreamWaitEvent(ctx->aux, ctx->ev_g, 0));
+      TENRSO_TRY(normal_eq_inverse(ctx, G, nd, mode, R, reg, Winv, scal, ctx->aux));
+      TENRSO_HIP_TRY(hipEventRecord(ctx->ev_w, ctx->aux));
+      MttkrpResult mr;
       TENRSO_TRY(mttkrp_dense(ctx, bf16 ? TENRSO_PREC_BF16 : TENRSO_PREC_FP32, Xk, Xt, nd, shape, Fc, R, mode,
-                              MttkrpOut::F64, M, nullptr));
-      TENRSO_TRY(solve_rows(ctx, M, I, R, Winv, epi, F[mode], ip_part, &nblk));
+                              MttkrpOut::PARTIAL, nullptr, &mr));
+      TENRSO_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_w, 0));
+      TENRSO_TRY(solve_rows(ctx, mr.partial, mr.splits, nullptr, I, R, Winv, epi, F[mode], ip_part, &nblk));
       TENRSO_TRY(gram_f64(ctx, F[mode], I, R, G + (i64)mode * R * R));
     }
     if (o->fit_from_mode0) {

@@ -18,16 +18,19 @@ namespace tenrso {
 
 static inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); }
 
-// block-wide sum of one double per thread (256 threads), fixed tree order
+// wave-wide sum (fixed butterfly order, every lane gets the total)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// block-wide sum of one double per thread (256 threads), fixed order
 __device__ double block_sum_256(double v, double* red) {
-  const int t = threadIdx.x;
-  red[t] = v;
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) red[t] += red[t + s];
-    __syncthreads();
-  }
-  double r = red[0];
+  const double r = (red[0] + red[1]) + (red[2] + red[3]);
   __syncthreads();
   return r;
 }
@@ -85,42 +88,62 @@ int tensor_norm_min(tenrso_ctx* ctx, const float* X, i64 n, double* tmp, double*
 }
 
 // ---------------------------------------------------------------- Gram A^T A
-// grid (ceil(R*R/256), rsplit): partial over a row range, then fixed-order reduce.
-__global__ __launch_bounds__(256) void gram_partial(const float* __restrict__ A, i64 rows, i64 R, int rsplit,
-                                                    double* __restrict__ part) {
-  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (e >= R * R) return;
-  const i64 r = e / R, s = e - (e / R) * R;
-  const i64 i0 = (i64)blockIdx.y * rows / rsplit, i1 = (i64)(blockIdx.y + 1) * rows / rsplit;
-  double acc = 0.0;
-  for (i64 i = i0; i < i1; ++i) acc += (double)A[i * R + r] * (double)A[i * R + s];
-  part[(i64)blockIdx.y * R * R + e] = acc;
-}
+// grid: ceil(R / 4) workgroups; workgroup b computes Gram rows 4b .. 4b+3 (all
+// columns).  Rows of A are staged through LDS in chunks of GRAM_CH rows with
+// 16-byte loads; each thread accumulates one (r, c) entry in FP64 over the
+// chunk order (deterministic).  R <= SOLVE_RMAX, R % 4 == 0 handled by masking.
+constexpr int GRAM_CH = 128;
 
-__global__ __launch_bounds__(256) void sum_parts(const double* __restrict__ part, int nparts, i64 n,
-                                                 double* __restrict__ out) {
-  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (e >= n) return;
-  double s = 0.0;
-  for (int p = 0; p < nparts; ++p) s += part[(i64)p * n + e];
-  out[e] = s;
+__global__ __launch_bounds__(256) void gram_rows_kernel(const float* __restrict__ A, i64 rows, int R,
+                                                        double* __restrict__ G) {
+  __shared__ __attribute__((aligned(16))) float sA[GRAM_CH * SOLVE_RMAX];
+  const int c = threadIdx.x & 63, rr = threadIdx.x >> 6;
+  const int r = blockIdx.x * 4 + rr;
+  double acc0 = 0.0, acc1 = 0.0;
+  for (i64 i0 = 0; i0 < rows; i0 += GRAM_CH) {
+    const int n = (int)((rows - i0) < GRAM_CH ? (rows - i0) : GRAM_CH);
+    const float* src = A + i0 * R;
+    const int tot = n * R;
+    __syncthreads();
+    if ((R & 3) == 0) {
+      for (int e = threadIdx.x * 4; e < tot; e += 1024)
+        *reinterpret_cast<float4*>(&sA[e]) = *reinterpret_cast<const float4*>(&src[e]);
+    } else {
+      for (int e = threadIdx.x; e < tot; e += 256) sA[e] = src[e];
+    }
+    __syncthreads();
+    if (r < R && c < R) {
+      int i = 0;
+      for (; i + 2 <= n; i += 2) {
+        acc0 += (double)sA[i * R + r] * (double)sA[i * R + c];
+        acc1 += (double)sA[(i + 1) * R + r] * (double)sA[(i + 1) * R + c];
+      }
+      if (i < n) acc0 += (double)sA[i * R + r] * (double)sA[i * R + c];
+    }
+  }
+  if (r < R && c < R) G[(i64)r * R + c] = acc0 + acc1;
 }
 
 int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G) {
-  const int rsplit = (int)std::max<i64>(1, std::min<i64>(64, rows / 64));
-  void* buf = nullptr;
-  TENRSO_TRY(ctx_buffer(ctx, SLOT_GRAM_PART, (size_t)rsplit * R * R * sizeof(double), &buf));
-  double* part = (double*)buf;
-  hipLaunchKernelGGL(gram_partial, dim3(cdiv(R * R, 256), rsplit), dim3(256), 0, ctx->stream, A, rows, R, rsplit,
-                     part);
-  hipLaunchKernelGGL(sum_parts, dim3(cdiv(R * R, 256)), dim3(256), 0, ctx->stream, part, rsplit, R * R, G);
+  if (R > SOLVE_RMAX) {
+    set_error("rank %lld exceeds the on-device Gram bound %d", (long long)R, SOLVE_RMAX);
+    return TENRSO_ERR_UNSUPPORTED;
+  }
+  if (R == 0) return TENRSO_OK;
+  hipLaunchKernelGGL(gram_rows_kernel, dim3(cdiv(R, 4)), dim3(256), 0, ctx->stream, A, rows, (int)R, G);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
 
 // ---------------------------------------------------------------- normal equations
 // One workgroup: Gamma = hadamard_{k != skip} G_k (ascending k, from ones) + reg,
-// then Gamma^{-1} by in-place Gauss-Jordan in LDS (R <= SOLVE_RMAX).
+// padded to 64 x 64 with the identity, inverted in place by Gauss-Jordan.
+// Thread t owns column c = t % 64 and rows rg + 4m (m < 16) in registers; each
+// pivot step publishes the pivot row and column through double-buffered LDS,
+// so one barrier per step suffices.  The output Winv is 64 x 64 (stride 64);
+// entries outside R x R are the identity and never used.
+constexpr int NE = 64;
+
 struct NormalEqArgs {
   const double* G;
   int nd, skip;
@@ -132,149 +155,200 @@ struct NormalEqArgs {
   double* scal;
 };
 
-__device__ void build_gamma(double* A, const NormalEqArgs& a, double ridge) {
+__device__ __forceinline__ double gamma_entry(const NormalEqArgs& a, int i, int j, double ridge) {
   const int R = a.R;
-  for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
-    double v = 1.0;
-    for (int k = 0; k < a.nd; ++k)
-      if (k != a.skip) v = v * a.G[(i64)k * R * R + e];
-    const int i = e / R, j = e - (e / R) * R;
-    if (a.tik_order > 0) {
-      // lambda * D^T D for first / second differences (cp/helpers.rs:630-702)
-      const double lam = a.tik_lambda;
-      const int n = R;
-      if (a.tik_order == 1) {
-        if (n >= 2) {
-          if (i == j) v += lam * ((i == 0 || i == n - 1) ? 1.0 : 2.0);
-          else if (j == i + 1 || i == j + 1) v -= lam;
-        }
-      } else {
-        if (n >= 3) {
-          if (i == j) v += lam * ((i == 0 || i == n - 1) ? 1.0 : ((i == 1 || i == n - 2) ? 5.0 : 6.0));
-          else if (j == i + 1 || i == j + 1) {
-            const int lo = i < j ? i : j;
-            v += lam * ((lo == 0 || lo == n - 2) ? -2.0 : -4.0);
-          } else if (j == i + 2 || i == j + 2) {
-            v += lam;
-          }
-        } else if (i == j) {
+  if (i >= R || j >= R) return i == j ? 1.0 : 0.0;
+  double v = 1.0;
+  for (int k = 0; k < a.nd; ++k)
+    if (k != a.skip) v = v * a.G[(i64)k * R * R + i * R + j];
+  if (a.tik_order > 0) {
+    // lambda * D^T D for first / second differences (cp/helpers.rs:630-702)
+    const double lam = a.tik_lambda;
+    const int n = R;
+    if (a.tik_order == 1) {
+      if (n >= 2) {
+        if (i == j) v += lam * ((i == 0 || i == n - 1) ? 1.0 : 2.0);
+        else if (j == i + 1 || i == j + 1) v -= lam;
+      }
+    } else {
+      if (n >= 3) {
+        if (i == j) v += lam * ((i == 0 || i == n - 1) ? 1.0 : ((i == 1 || i == n - 2) ? 5.0 : 6.0));
+        else if (j == i + 1 || i == j + 1) {
+          const int lo = i < j ? i : j;
+          v += lam * ((lo == 0 || lo == n - 2) ? -2.0 : -4.0);
+        } else if (j == i + 2 || i == j + 2) {
           v += lam;
         }
+      } else if (i == j) {
+        v += lam;
       }
-    } else if (a.l2 > 0.0 && i == j) {
-      v += a.l2;
     }
-    if (i == j) v += ridge;
-    A[e] = v;
+  } else if (a.l2 > 0.0 && i == j) {
+    v += a.l2;
   }
+  if (i == j) v += ridge;
+  return v;
 }
 
-// Gauss-Jordan inverse in place; returns false on a non-positive or non-finite pivot.
-__device__ bool gauss_jordan_spd(double* A, int R) {
-  constexpr int MAXE = SOLVE_RMAX * SOLVE_RMAX / 256;
-  double tmp[MAXE];
-  for (int j = 0; j < R; ++j) {
-    __syncthreads();
-    const double p = A[j * R + j];
-    if (!(p > 0.0) || !isfinite(p)) return false;  // uniform: every thread reads the same p
-    const double ip = 1.0 / p;
+// FP64 reciprocal: hardware estimate refined by two Newton steps.
+__device__ __forceinline__ double recip(double p) {
+  double r = __builtin_amdgcn_rcp(p);
+  r = fma(r, fma(-p, r, 1.0), r);
+  r = fma(r, fma(-p, r, 1.0), r);
+  return r;
+}
+
+// In-place Gauss-Jordan inverse of the 64 x 64 matrix held as av[m] = A[rg+4m][c].
+// Step j: the wave owning row j (rg == j % 4) reads the pivot with readlane,
+// publishes 1/p and the scaled pivot row; the four threads with c == j publish
+// the pivot column; one barrier; every thread updates its 16 entries.  A bad
+// pivot (<= 0 or not finite) clears `ok` instead of branching, so every thread
+// runs the same 64 steps.
+__device__ bool gj_invert(double (&av)[16], int c, int rg, double* prow, double* pcol, double* pinv) {
+  bool ok = true;
+  for (int j = 0; j < NE; ++j) {
+    const int b = j & 1;
+    if (rg == (j & 3)) {
+      // pv = av[j / 4] without a dynamic register index
+      double pv = 0.0;
 #pragma unroll
-    for (int m = 0; m < MAXE; ++m) {
-      const int e = threadIdx.x + 256 * m;
-      if (e < R * R) {
-        const int i = e / R, c = e - (e / R) * R;
-        const double aij = A[i * R + j], ajc = A[j * R + c], aic = A[e];
-        double v;
-        if (i == j && c == j) v = ip;
-        else if (i == j) v = ajc * ip;
-        else if (c == j) v = -aij * ip;
-        else v = aic - aij * ajc * ip;
-        tmp[m] = v;
-      }
+      for (int m = 0; m < 16; ++m) pv = (m == (j >> 2)) ? av[m] : pv;
+      const int lo = __builtin_amdgcn_readlane(__double2loint(pv), j);
+      const int hi = __builtin_amdgcn_readlane(__double2hiint(pv), j);
+      const double p = __hiloint2double(hi, lo);
+      const double ip = recip(p);
+      prow[b * NE + c] = pv * ip;
+      if (c == 0) pinv[b] = (p > 0.0 && isfinite(p)) ? ip : __longlong_as_double(0x7ff8000000000000LL);
+    }
+    if (c == j) {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) pcol[b * NE + rg + 4 * m] = av[m];
     }
     __syncthreads();
+    const double ip = pinv[b];
+    ok = ok && !isnan(ip);
+    const double ajc = prow[b * NE + c];
 #pragma unroll
-    for (int m = 0; m < MAXE; ++m) {
-      const int e = threadIdx.x + 256 * m;
-      if (e < R * R) A[e] = tmp[m];
+    for (int m = 0; m < 16; ++m) {
+      const int i = rg + 4 * m;
+      const double aij = pcol[b * NE + i];
+      const double upd = fma(-aij, ajc, av[m]);
+      const double colv = -aij * ip;
+      double v = (c == j) ? colv : upd;
+      if (i == j) v = (c == j) ? ip : ajc;
+      av[m] = v;
     }
   }
   __syncthreads();
-  return true;
+  return ok;
 }
 
 __global__ __launch_bounds__(256) void normal_eq_kernel(NormalEqArgs a) {
-  __shared__ double A[SOLVE_RMAX * SOLVE_RMAX];
-  const int R = a.R;
+  __shared__ double prow[2 * NE], pcol[2 * NE], pinv[2];
+  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  double av[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) av[m] = gamma_entry(a, rg + 4 * m, c, 0.0);
   int path = 0;
-  build_gamma(A, a, 0.0);
-  bool ok = gauss_jordan_spd(A, R);
+  bool ok = gj_invert(av, c, rg, prow, pcol, pinv);
   if (!ok) {
     path = 1;
-    __syncthreads();
-    build_gamma(A, a, (double)FLT_EPSILON * (double)(10 * R));
-    ok = gauss_jordan_spd(A, R);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) av[m] = gamma_entry(a, rg + 4 * m, c, (double)FLT_EPSILON * (double)(10 * a.R));
+    ok = gj_invert(av, c, rg, prow, pcol, pinv);
     if (!ok) path = 2;
   }
-  for (int e = threadIdx.x; e < R * R; e += 256) a.Winv[e] = ok ? A[e] : 0.0;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) a.Winv[(rg + 4 * m) * NE + c] = ok ? av[m] : 0.0;
   if (threadIdx.x == 0 && a.scal) a.scal[SC_STATUS] = fmax(a.scal[SC_STATUS], (double)path);
 }
 
 int normal_eq_inverse(tenrso_ctx* ctx, const double* G, int nd, int skip, i64 R, const GramReg& reg,
-                      double* Winv, double* scal) {
+                      double* Winv, double* scal, hipStream_t stream) {
   if (R > SOLVE_RMAX) {
     set_error("rank %lld exceeds the on-device normal-equation solve bound %d", (long long)R, SOLVE_RMAX);
     return TENRSO_ERR_UNSUPPORTED;
   }
   NormalEqArgs a{G, nd, skip, (int)R, reg.l2, reg.tikhonov_order, reg.tikhonov_lambda, Winv, scal};
-  hipLaunchKernelGGL(normal_eq_kernel, dim3(1), dim3(256), 0, ctx->stream, a);
+  hipLaunchKernelGGL(normal_eq_kernel, dim3(1), dim3(256), 0, stream ? stream : ctx->stream, a);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
 
 // ---------------------------------------------------------------- row solve
-// grid: ceil(rows / 16); 256 threads = 4 row groups x 64 columns.
-constexpr int SOLVE_RB = 16;
+// grid: rows (one workgroup per row); 256 threads = 4 split quarters x 64
+// columns.  Each thread sums a quarter of the MTTKRP split partials of its
+// column (all loads in flight, FP64), the quarters are added in fixed order,
+// then A[i, c] = epi(sum_r M[i, r] Winv[r, c]) (the r-sum split over the same
+// four thread groups) and the block's <M, A> partial is written.
+constexpr int SOLVE_RB = 1;
 
-__global__ __launch_bounds__(256) void solve_rows_kernel(const double* __restrict__ M, i64 rows, int R,
+__global__ __launch_bounds__(256) void solve_rows_kernel(const float* __restrict__ part, int splits,
+                                                         const double* __restrict__ M64, i64 rows, int R,
                                                          const double* __restrict__ Winv, double l1, int nonneg,
                                                          float* __restrict__ A, double* __restrict__ ip_part) {
-  __shared__ double sM[SOLVE_RB * SOLVE_RMAX];
-  __shared__ double red[256];
-  const i64 r0 = (i64)blockIdx.x * SOLVE_RB;
-  const int nrows = (int)((rows - r0) < SOLVE_RB ? (rows - r0) : SOLVE_RB);
-  for (int e = threadIdx.x; e < nrows * R; e += 256) sM[e] = M[r0 * R + e];
-  __syncthreads();
-  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
-  double ip = 0.0;
-  for (int cc = c; cc < R; cc += 64) {
-    double acc[SOLVE_RB / 4];
-#pragma unroll
-    for (int k = 0; k < SOLVE_RB / 4; ++k) acc[k] = 0.0;
-    for (int r = 0; r < R; ++r) {
-      const double w = Winv[r * R + cc];
-#pragma unroll
-      for (int k = 0; k < SOLVE_RB / 4; ++k) acc[k] += sM[(rg + 4 * k) * R + r] * w;
-    }
-#pragma unroll
-    for (int k = 0; k < SOLVE_RB / 4; ++k) {
-      const int lr = rg + 4 * k;
-      if (lr < nrows) {
-        double x = acc[k];
-        if (l1 > 0.0) x = x > l1 ? x - l1 : (x < -l1 ? x + l1 : 0.0);  // soft_threshold (helpers.rs:610-623)
-        if (nonneg) x = x > 0.0 ? x : 0.0;                               // core.rs:357-360
-        const float x32 = (float)x;
-        A[(r0 + lr) * R + cc] = x32;
-        ip += sM[lr * R + cc] * (double)x32;
+  __shared__ double sq[4 * NE];
+  __shared__ double sM[NE];
+  __shared__ double red[4];
+  const int c = threadIdx.x & 63, qd = threadIdx.x >> 6;
+  const i64 row = blockIdx.x;
+  const bool colok = c < R;
+  double m = 0.0;
+  if (colok) {
+    if (part) {
+      const i64 n = rows * R, off = row * R + c;
+      const int s0 = qd * splits / 4, s1 = (qd + 1) * splits / 4;
+      double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+      int sp = s0;
+      for (; sp + 4 <= s1; sp += 4) {
+        a0 += (double)part[(i64)(sp + 0) * n + off];
+        a1 += (double)part[(i64)(sp + 1) * n + off];
+        a2 += (double)part[(i64)(sp + 2) * n + off];
+        a3 += (double)part[(i64)(sp + 3) * n + off];
       }
+      for (; sp < s1; ++sp) a0 += (double)part[(i64)sp * n + off];
+      m = (a0 + a1) + (a2 + a3);
+    } else if (qd == 0) {
+      m = M64[row * R + c];
     }
   }
-  const double tot = block_sum_256(ip, red);
-  if (threadIdx.x == 0 && ip_part) ip_part[blockIdx.x] = tot;
+  sq[qd * NE + c] = m;
+  __syncthreads();
+  if (qd == 0) sM[c] = (sq[c] + sq[NE + c]) + (sq[2 * NE + c] + sq[3 * NE + c]);
+  __syncthreads();
+  // x[c] = sum_r M[r] Winv[r, c], r split over the four quarters
+  double x = 0.0;
+  if (colok) {
+    const int r0 = qd * R / 4, r1 = (qd + 1) * R / 4;
+    double x0 = 0, x1 = 0;
+    int r = r0;
+    for (; r + 2 <= r1; r += 2) {
+      x0 += sM[r] * Winv[r * NE + c];
+      x1 += sM[r + 1] * Winv[(r + 1) * NE + c];
+    }
+    if (r < r1) x0 += sM[r] * Winv[r * NE + c];
+    x = x0 + x1;
+  }
+  sq[qd * NE + c] = x;
+  __syncthreads();
+  double ip = 0.0;
+  if (qd == 0 && colok) {
+    double xv = (sq[c] + sq[NE + c]) + (sq[2 * NE + c] + sq[3 * NE + c]);
+    if (l1 > 0.0) xv = xv > l1 ? xv - l1 : (xv < -l1 ? xv + l1 : 0.0);  // soft_threshold (helpers.rs:610-623)
+    if (nonneg) xv = xv > 0.0 ? xv : 0.0;                                  // core.rs:357-360
+    const float x32 = (float)xv;
+    A[row * R + c] = x32;
+    ip = sM[c] * (double)x32;
+  }
+  if (qd == 0) {
+    ip = wave_sum(ip);
+    if (c == 0 && ip_part) ip_part[blockIdx.x] = ip;
+  }
+  (void)red;
 }
 
-int solve_rows(tenrso_ctx* ctx, const double* M, i64 rows, i64 R, const double* Winv, const RowEpilogue& epi,
-               float* A, double* ip_part, int* nblk_out) {
+int solve_rows(tenrso_ctx* ctx, const float* part, int splits, const double* M, i64 rows, i64 R,
+               const double* Winv, const RowEpilogue& epi, float* A, double* ip_part, int* nblk_out) {
   const unsigned nblk = cdiv(rows, SOLVE_RB);
   if (nblk_out) *nblk_out = (int)nblk;
   if (rows == 0) return TENRSO_OK;
@@ -282,8 +356,8 @@ int solve_rows(tenrso_ctx* ctx, const double* M, i64 rows, i64 R, const double* 
     set_error("rank %lld exceeds the on-device solve bound %d", (long long)R, SOLVE_RMAX);
     return TENRSO_ERR_UNSUPPORTED;
   }
-  hipLaunchKernelGGL(solve_rows_kernel, dim3(nblk), dim3(256), 0, ctx->stream, M, rows, (int)R, Winv, epi.l1,
-                     epi.nonneg, A, ip_part);
+  hipLaunchKernelGGL(solve_rows_kernel, dim3(nblk), dim3(256), 0, ctx->stream, part, splits, M, rows, (int)R, Winv,
+                     epi.l1, epi.nonneg, A, ip_part);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -312,7 +386,7 @@ int inner_rows(tenrso_ctx* ctx, const double* M, const float* A, i64 rows, i64 R
 __global__ __launch_bounds__(256) void fit_kernel(const double* __restrict__ G, int nd, int R,
                                                   const double* __restrict__ ip_part, int nblk,
                                                   double* __restrict__ scal) {
-  __shared__ double red[256];
+  __shared__ double red[4];
   // ||Xhat||^2 = sum_{r,s} prod_k G_k[r,s]   (cp/helpers.rs:533-555)
   double rec = 0.0;
   for (int e = threadIdx.x; e < R * R; e += 256) {

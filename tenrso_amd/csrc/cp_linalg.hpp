@@ -34,13 +34,15 @@ enum Scalars : int {
 int tensor_norm_min(tenrso_ctx* ctx, const float* X, i64 n, double* tmp, double* scal);
 // G = A^T A (FP64) for an fp32 rows x R factor.
 int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G);
-// Winv = (hadamard_{k != skip} G_k + reg)^{-1}; G: [nd][R][R].  Writes the path taken
-// into scal[SC_STATUS] (max with the previous value).
+// Winv = (hadamard_{k != skip} G_k + reg)^{-1}, 64 x 64 (stride 64, identity padding);
+// G: [nd][R][R].  Writes the path taken into scal[SC_STATUS] (max with the previous
+// value).  Runs on `stream` (null: the context stream).
 int normal_eq_inverse(tenrso_ctx* ctx, const double* G, int nd, int skip, i64 R, const GramReg& reg,
-                      double* Winv, double* scal);
-// A[i,:] = epi(M[i,:] Winv) for an fp64 M; ip_part[blk] = sum_i M[i,:].A[i,:] (FP64).
-int solve_rows(tenrso_ctx* ctx, const double* M, i64 rows, i64 R, const double* Winv, const RowEpilogue& epi,
-               float* A, double* ip_part, int* nblk_out);
+                      double* Winv, double* scal, hipStream_t stream);
+// A[i,:] = epi(M[i,:] Winv), M = sum of the `splits` fp32 partials [splits][rows][R]
+// (or the fp64 M when part is null); ip_part[blk] = sum_i M[i,:].A[i,:] (FP64).
+int solve_rows(tenrso_ctx* ctx, const float* part, int splits, const double* M, i64 rows, i64 R,
+               const double* Winv, const RowEpilogue& epi, float* A, double* ip_part, int* nblk_out);
 // ip_part[blk] = sum_{i in blk} M[i,:] . A[i,:]
 int inner_rows(tenrso_ctx* ctx, const double* M, const float* A, i64 rows, i64 R, double* ip_part, int* nblk_out);
 // fit from the Grams, the ip partials and ||X||^2 (cp/helpers.rs:513-555).

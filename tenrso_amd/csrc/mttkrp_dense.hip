@@ -9,11 +9,13 @@
 //   * X is viewed in place as [P, I, Q] (P = prod of modes before n, Q = after).
 //     The contraction index j = (p, q) runs over the other modes in forward order,
 //     so j = p*Q + q and X[p, i, q] sits at (p*I + i)*Q + q.
-//   * j is cut into blocks of KB consecutive indices of the fastest other mode.
-//     Within a block KR[j, r] = PRE[j / F_fast, r] * FAST[j % F_fast, r], where
-//     FAST is the factor of the fastest other mode and PRE the forward KR of the
-//     remaining other modes (a factor itself for 3-way tensors).  Each workgroup
-//     writes the KR block, transposed to [r][j], into LDS (double buffered).
+//   * j = (slow, k): k runs over the fastest other mode, slow over the rest, and
+//     KR[j, r] = PRE[slow, r] * FAST[k, r], where FAST is the factor of the
+//     fastest other mode and PRE the forward KR of the remaining other modes (a
+//     factor itself for 3-way tensors).  A workgroup stages a window of FAST,
+//     transposed to [r][k], in LDS once; for each slow index it contracts
+//     T[i, r] = sum_k X[.., i, .., k] FAST[k, r] on MFMA and adds PRE[slow, r] T[i, r]
+//     into its accumulator, so no Khatri-Rao row is ever written anywhere.
 //   * Rows i are the MFMA M dimension, rank columns r the N dimension, j the K
 //     dimension.  FP32 uses v_mfma_f32_16x16x4_f32; BF16 uses
 //     v_mfma_f32_16x16x32_bf16.  For q-contiguous X (mode < N-1, "case A") each
@@ -31,17 +33,21 @@ namespace tenrso {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx8 __attribute__((ext_vector_type(8)));
 
 struct KernArgs {
-  i64 I, Q;          // X viewed as [P, I, Q]
-  i64 R;             // factor row stride (rank)
-  i64 fast_size;     // size of the fastest other mode
-  const float* pre;  // PRE (rows = J / fast_size, stride R) or null (= 1)
-  const float* fast; // FAST factor (fast_size x R)
-  i64 nblocks;       // J / KB
-  int splits;
-  int r0;            // first rank column of this launch (grid.z chunk * 16*NT)
-  int rcount;        // rank columns handled by the launch (<= 16*NT)
+  i64 I, Q;           // X viewed as [P, I, Q]
+  i64 R;              // factor row stride (rank)
+  i64 fast_size;      // size of the fastest other mode
+  i64 qs;             // Q / fast_size: slow indices per p (case A)
+  const float* pre;   // PRE (rows = J / fast_size, stride R) or null (= 1)
+  const float* fast;  // FAST factor (fast_size x R)
+  int kw;             // window of the fastest mode held in LDS (multiple of KB, divides fast_size)
+  int nwin;           // fast_size / kw
+  i64 n_slow;         // J / fast_size
+  int slow_splits;    // splits of the slow range; grid.y = nwin * slow_splits
+  int r0;             // first rank column of this launch
+  int rcount;         // rank columns handled by the launch (<= 16*NT)
 };
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
@@ -51,216 +57,359 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
   return (uint16_t)(u >> 16);
 }
 
-// KR block (rows j0 .. j0+KB-1, columns r0 .. r0+RT-1) into LDS, transposed [r][jj].
-template <int RT, int KB, int PITCH, int NTH, typename ST>
-__device__ __forceinline__ void gen_kr_block(ST* __restrict__ s, const KernArgs& a, i64 j0, int tid) {
-  const i64 slow = j0 / a.fast_size;
-  const i64 fast0 = j0 - slow * a.fast_size;
-  static_assert(NTH % RT == 0, "thread count must be a multiple of the rank tile");
-  const int r = tid % RT;
-  const int jstart = tid / RT;
-  const bool valid = r < a.rcount;
-  const int rc = valid ? a.r0 + r : a.r0;
-  const float pre = (valid && a.pre) ? a.pre[slow * a.R + rc] : 1.0f;
-  const float* fp = a.fast + fast0 * a.R + rc;
-#pragma unroll
-  for (int jj = jstart; jj < KB; jj += NTH / RT) {
-    float v = valid ? pre * fp[(i64)jj * a.R] : 0.0f;
-    if constexpr (sizeof(ST) == 4) {
-      s[r * PITCH + jj] = v;
-    } else {
-      s[r * PITCH + jj] = f32_to_bf16_rne(v);
-    }
+// Stage FAST[w*kw .. w*kw+kw) (columns r0 .. r0+RT) into LDS transposed: s[r][kk].
+// Columns past rcount are zero, so they contribute nothing.
+template <int RT, int PITCH, int NTH>
+__device__ __forceinline__ void stage_fast_window(float* __restrict__ s, const KernArgs& a, int w) {
+  const i64 k0 = (i64)w * a.kw;
+  for (int idx = threadIdx.x; idx < RT * a.kw; idx += NTH) {
+    const int r = idx % RT, kk = idx / RT;
+    s[r * PITCH + kk] = (r < a.rcount) ? a.fast[(k0 + kk) * a.R + a.r0 + r] : 0.0f;
   }
 }
 
-// ---------------------------------------------------------------------------
-// FP32 MFMA kernel.  NT: rank tiles of 16 (RT = 16*NT <= 64).  WAVES: waves per
-// workgroup, each owning MT m-tiles of 16 rows.  KB = 32 contraction indices per
-// block: 2 halves (h) x 4 lane groups (g) x 4 consecutive (s).
-// ---------------------------------------------------------------------------
-template <int NT, int WAVES, int MT, bool CASE_B>
-__global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __restrict__ X, KernArgs a,
-                                                              float* __restrict__ partial) {
-  constexpr int KB = 32, PITCH = KB + 4, RT = 16 * NT, NTH = WAVES * 64, ROWS = WAVES * 16 * MT;
-  __shared__ __attribute__((aligned(16))) float sKR[2][RT * PITCH];
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int g = lane >> 4, c = lane & 15;
-  const int split = blockIdx.y;
-  const i64 b_begin = (i64)split * a.nblocks / a.splits;
-  const i64 b_end = (i64)(split + 1) * a.nblocks / a.splits;
-  const i64 row_base = (i64)blockIdx.x * ROWS + wave * 16 * MT;
-
-  i64 rowc[MT];
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-    i64 rr = row_base + mt * 16 + c;
-    rowc[mt] = rr < a.I ? rr : a.I - 1;  // clamp: rows past I load valid data, are never stored
+// Same window as an exact bf16 pair: FAST = hi + lo with hi = bf16(FAST) and
+// lo = bf16(FAST - hi), so hi + lo carries 16 significant bits.
+template <int RT, int PITCH, int NTH>
+__device__ __forceinline__ void stage_fast_window_bf16x2(uint16_t* __restrict__ shi, uint16_t* __restrict__ slo,
+                                                         const KernArgs& a, int w) {
+  const i64 k0 = (i64)w * a.kw;
+  for (int idx = threadIdx.x; idx < RT * a.kw; idx += NTH) {
+    const int r = idx % RT, kk = idx / RT;
+    const float v = (r < a.rcount) ? a.fast[(k0 + kk) * a.R + a.r0 + r] : 0.0f;
+    const uint16_t h = f32_to_bf16_rne(v);
+    const float vh = __uint_as_float((uint32_t)h << 16);
+    shi[r * PITCH + kk] = h;
+    slo[r * PITCH + kk] = f32_to_bf16_rne(v - vh);
   }
+}
 
-  floatx4 acc[MT][NT];
+// Fixed-order reduction of the WAVES per-wave accumulators of a workgroup into
+// one [ROWS x RT] partial (waves 1.. stage through LDS, wave 0 adds in order).
+template <int MT, int NT, int WAVES>
+__device__ __forceinline__ void wg_reduce_acc(floatx4 (&acc)[MT][NT], float* __restrict__ red, int wave, int lane) {
+  constexpr int PER = MT * NT * 4;  // floats per lane
+  __syncthreads();                  // LDS window no longer read
+  if (wave > 0) {
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[((wave - 1) * PER + (mt * NT + nt) * 4 + v) * 64 + lane] = acc[mt][nt][v];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int wv = 1; wv < WAVES; ++wv)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[mt][nt][v] += red[((wv - 1) * PER + (mt * NT + nt) * 4 + v) * 64 + lane];
+  }
+}
+
+// Fold the per-slow accumulator T into acc with the PRE row of that slow index:
+// acc[:, r] += PRE[slow, r] * T[:, r]  (lane (g, c) holds column 16nt + c).
+template <int MT, int NT>
+__device__ __forceinline__ void fold_slow(floatx4 (&acc)[MT][NT], floatx4 (&T)[MT][NT], const float (&pre)[NT]) {
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int nt = 0; nt < NT; ++nt) {
+      acc[mt][nt] += T[mt][nt] * pre[nt];
+      T[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+}
 
-  // xa[mt][h] holds j = j0 + 16h + 4g + s, s = 0..3
-  auto loadx = [&](i64 b, floatx4 (&xa)[MT][2]) {
-    const i64 j0 = b * KB;
+// ---------------------------------------------------------------------------
+// FP32 MFMA kernel (v_mfma_f32_16x16x4f32).
+//   A workgroup owns ROWS = 16*MT rows and one window w of the fastest mode; its
+//   WAVES waves share the LDS window and split the workgroup's slow range
+//   (wave k takes slow indices s_lo + k, s_lo + k + WAVES, ...); their
+//   accumulators are summed in LDS in fixed order at the end.
+//   For one slow index the contraction over the window is
+//     T[i, r] = sum_k X[.., i, .., k] * FAST[k, r]
+//   on MFMA (B operand = the FAST window, read as 4 consecutive k from LDS), and
+//   then acc[i, r] += PRE[slow, r] * T[i, r], which together is
+//   sum_j X_(n)[i, j] KR[j, r] with KR[j, r] = PRE[slow, r] * FAST[k, r].
+//   One block = KB = 32 consecutive j: 2 halves h x 4 lane groups g x 4 s, so
+//   MFMA step (h, s) of lane group g contracts j = j0 + 16h + 4g + s.
+//   Case A (q contiguous): lane (g, c) of m-tile mt holds row row_base+16mt+c and
+//     loads 4 consecutive q with one 16-byte load per h.
+//   Case B (i contiguous, last mode): rows are interleaved, lane c of m-tile mt
+//     holds row row_base + MT*c + mt, so one MT-wide vector load per j covers
+//     16*MT consecutive rows.
+//   X blocks are prefetched 3 blocks ahead in registers.
+// ---------------------------------------------------------------------------
+template <int NT, int WAVES, int MT, bool CASE_B, int KWMAX>
+__global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __restrict__ X, KernArgs a,
+                                                              float* __restrict__ partial) {
+  constexpr int KB = 32, PITCH = KWMAX + 4, RT = 16 * NT, NTH = WAVES * 64, ROWS = 16 * MT;
+  constexpr int LDS_WIN = RT * PITCH, LDS_RED = (WAVES - 1) * MT * NT * 4 * 64;
+  __shared__ __attribute__((aligned(16))) float smem[LDS_WIN > LDS_RED ? LDS_WIN : LDS_RED];
+  float* sF = smem;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int g = lane >> 4, c = lane & 15;
+  const int split = blockIdx.y;
+  const int w = split % a.nwin;
+  const int ss = split / a.nwin;
+  const i64 s_lo = (i64)ss * a.n_slow / a.slow_splits;
+  const i64 s_hi = (i64)(ss + 1) * a.n_slow / a.slow_splits;
+  const int nkb = a.kw / KB;
+  const i64 my_slow = (s_hi - s_lo > wave) ? (s_hi - s_lo - wave + WAVES - 1) / WAVES : 0;
+  const i64 T = my_slow * nkb;
+  const i64 row_base = (i64)blockIdx.x * ROWS;
+
+  stage_fast_window<RT, PITCH, NTH>(sF, a, w);
+
+  i64 rowc[MT];
+  i64 rowv = 0;  // case B: first row of this lane's MT-vector
+  if constexpr (!CASE_B) {
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const i64 rr = row_base + 16 * mt + c;
+      rowc[mt] = rr < a.I ? rr : a.I - 1;
+    }
+  } else {
+    const i64 rr = row_base + MT * c;
+    rowv = rr <= a.I - MT ? rr : a.I - MT;
+  }
+
+  floatx4 acc[MT][NT], Tacc[MT][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+      Tacc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
+
+  struct Buf {
+    floatx4 x[MT][2];
+    float pre[NT];
+    int kb;
+  };
+
+  auto load = [&](i64 t, Buf& b) {
+    if (t >= T) return;
+    const int tq = (int)t / nkb;
+    const int kb = (int)t - tq * nkb;
+    const i64 slow = s_lo + wave + (i64)tq * WAVES;
+    b.kb = kb;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int r = 16 * nt + c;
+      b.pre[nt] = (a.pre && r < a.rcount) ? a.pre[slow * a.R + a.r0 + r] : 1.0f;
+    }
     if constexpr (!CASE_B) {
-      const i64 p = j0 / a.Q, q0 = j0 - p * a.Q;
+      const int sl32 = (int)slow, qs32 = (int)a.qs;
+      const i64 p = sl32 / qs32;
+      const i64 q0 = (i64)(sl32 - (int)p * qs32) * a.fast_size + (i64)w * a.kw + kb * KB;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         const float* base = X + (p * a.I + rowc[mt]) * a.Q + q0 + 4 * g;
-        xa[mt][0] = *reinterpret_cast<const floatx4*>(base);
-        xa[mt][1] = *reinterpret_cast<const floatx4*>(base + 16);
+        b.x[mt][0] = *reinterpret_cast<const floatx4*>(base);
+        b.x[mt][1] = *reinterpret_cast<const floatx4*>(base + 16);
       }
     } else {
+      const i64 j0 = slow * a.fast_size + (i64)w * a.kw + kb * KB + 4 * g;
+      const float* base = X + j0 * a.I + rowv;
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        const float* base = X + (j0 + 4 * g) * a.I + rowc[mt];
+      for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          floatx4 v;
-          v[0] = base[(16 * h + 0) * a.I];
-          v[1] = base[(16 * h + 1) * a.I];
-          v[2] = base[(16 * h + 2) * a.I];
-          v[3] = base[(16 * h + 3) * a.I];
-          xa[mt][h] = v;
+        for (int sidx = 0; sidx < 4; ++sidx) {
+          const float* p = base + (i64)(16 * h + sidx) * a.I;
+          if constexpr (MT == 1) {
+            b.x[0][h][sidx] = p[0];
+          } else if constexpr (MT == 2) {
+            const float2 v = *reinterpret_cast<const float2*>(p);
+            b.x[0][h][sidx] = v.x;
+            b.x[1][h][sidx] = v.y;
+          } else {
+            const floatx4 v = *reinterpret_cast<const floatx4*>(p);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) b.x[mt][h][sidx] = v[mt];
+          }
         }
-      }
     }
   };
 
-  floatx4 xa[MT][2], xn[MT][2];
-  if (b_begin < b_end) {
-    gen_kr_block<RT, KB, PITCH, NTH>(sKR[0], a, b_begin * KB, tid);
-    loadx(b_begin, xa);
-  }
-  __syncthreads();
-  for (i64 b = b_begin; b < b_end; ++b) {
-    const int buf = (int)((b - b_begin) & 1);
-    if (b + 1 < b_end) {
-      loadx(b + 1, xn);
-      gen_kr_block<RT, KB, PITCH, NTH>(sKR[buf ^ 1], a, (b + 1) * KB, tid);
-    }
-    const float* s = sKR[buf];
+  auto compute = [&](i64 t, const Buf& b) {
+    if (t >= T) return;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
+      floatx4 bv[NT];
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        const floatx4 bv = *reinterpret_cast<const floatx4*>(&s[(16 * nt + c) * PITCH + 16 * h + 4 * g]);
+      for (int nt = 0; nt < NT; ++nt)
+        bv[nt] = *reinterpret_cast<const floatx4*>(&sF[(16 * nt + c) * PITCH + b.kb * KB + 16 * h + 4 * g]);
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[mt][h][0], bv[0], acc[mt][nt], 0, 0, 0);
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[mt][h][1], bv[1], acc[mt][nt], 0, 0, 0);
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[mt][h][2], bv[2], acc[mt][nt], 0, 0, 0);
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[mt][h][3], bv[3], acc[mt][nt], 0, 0, 0);
-        }
-      }
+      for (int sidx = 0; sidx < 4; ++sidx)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt)
+            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(b.x[mt][h][sidx], bv[nt][sidx], Tacc[mt][nt], 0, 0, 0);
     }
-    __syncthreads();
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      xa[mt][0] = xn[mt][0];
-      xa[mt][1] = xn[mt][1];
-    }
+    if (b.kb == nkb - 1) fold_slow<MT, NT>(acc, Tacc, b.pre);
+  };
+
+  Buf b0, b1, b2, b3;
+  load(0, b0);
+  load(1, b1);
+  load(2, b2);
+  __syncthreads();  // LDS window staged
+  for (i64 t = 0; t < T; t += 4) {
+    load(t + 3, b3);
+    compute(t, b0);
+    load(t + 4, b0);
+    compute(t + 1, b1);
+    load(t + 5, b1);
+    compute(t + 2, b2);
+    load(t + 6, b2);
+    compute(t + 3, b3);
   }
 
-  // D layout (16x16): lane (g, c), register v -> row 4g + v, column c
+  wg_reduce_acc<MT, NT, WAVES>(acc, smem, wave, lane);
+  if (wave != 0) return;
+  // D layout (16x16): lane (g, c), register v -> tile row 4g + v, column c
   float* out = partial + (i64)split * a.I * a.R;
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-      const int col = a.r0 + 16 * nt + c;
       if (16 * nt + c >= a.rcount) continue;
+      const int col = a.r0 + 16 * nt + c;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const i64 row = row_base + mt * 16 + 4 * g + v;
+        const i64 row = CASE_B ? row_base + MT * (4 * g + v) + mt : row_base + 16 * mt + 4 * g + v;
         if (row < a.I) out[row * a.R + col] = acc[mt][nt][v];
       }
     }
 }
 
 // ---------------------------------------------------------------------------
-// BF16 MFMA kernel (case A only: q-contiguous X).  KB = 32*KS contraction indices
-// per block; each lane holds 8 consecutive j per k-step (16-byte loads).
+// BF16 MFMA kernel (v_mfma_f32_16x16x32_bf16), case A only (q-contiguous X; the
+// last mode runs on the last-to-front copy).  Same work split and the same
+// T-then-PRE contraction as the FP32 kernel.  X enters as bf16 (one 16-byte load
+// of 8 consecutive j per lane and k-step); the FAST window enters as an exact
+// bf16 hi + lo pair (two MFMAs per step), so the input tensor is the only
+// operand rounded to bf16.  Accumulation is FP32.
 // ---------------------------------------------------------------------------
-template <int NT, int WAVES, int MT, int KS>
+template <int NT, int WAVES, int MT, int KS, int KWMAX>
 __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* __restrict__ X, KernArgs a,
                                                                float* __restrict__ partial) {
-  constexpr int KB = 32 * KS, PITCH = KB + 8, RT = 16 * NT, NTH = WAVES * 64, ROWS = WAVES * 16 * MT;
-  __shared__ __attribute__((aligned(16))) uint16_t sKR[2][RT * PITCH];
+  constexpr int KB = 32 * KS, PITCH = KWMAX + 8, RT = 16 * NT, NTH = WAVES * 64, ROWS = 16 * MT;
+  constexpr int LDS_WIN_U16 = 2 * RT * PITCH, LDS_RED_F = (WAVES - 1) * MT * NT * 4 * 64;
+  constexpr int LDS_BYTES = (LDS_WIN_U16 * 2 > LDS_RED_F * 4) ? LDS_WIN_U16 * 2 : LDS_RED_F * 4;
+  __shared__ __attribute__((aligned(16))) float smem[LDS_BYTES / 4];
+  uint16_t* sHi = reinterpret_cast<uint16_t*>(smem);
+  uint16_t* sLo = sHi + RT * PITCH;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = lane >> 4, c = lane & 15;
   const int split = blockIdx.y;
-  const i64 b_begin = (i64)split * a.nblocks / a.splits;
-  const i64 b_end = (i64)(split + 1) * a.nblocks / a.splits;
-  const i64 row_base = (i64)blockIdx.x * ROWS + wave * 16 * MT;
+  const int w = split % a.nwin;
+  const int ss = split / a.nwin;
+  const i64 s_lo = (i64)ss * a.n_slow / a.slow_splits;
+  const i64 s_hi = (i64)(ss + 1) * a.n_slow / a.slow_splits;
+  const int nkb = a.kw / KB;
+  const i64 my_slow = (s_hi - s_lo > wave) ? (s_hi - s_lo - wave + WAVES - 1) / WAVES : 0;
+  const i64 T = my_slow * nkb;
+  const i64 row_base = (i64)blockIdx.x * ROWS;
+
+  stage_fast_window_bf16x2<RT, PITCH, NTH>(sHi, sLo, a, w);
 
   i64 rowc[MT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) {
-    i64 rr = row_base + mt * 16 + c;
+    const i64 rr = row_base + 16 * mt + c;
     rowc[mt] = rr < a.I ? rr : a.I - 1;
   }
-  floatx4 acc[MT][NT];
+  floatx4 acc[MT][NT], Tacc[MT][NT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int nt = 0; nt < NT; ++nt) {
+      acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+      Tacc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+    }
 
-  auto loadx = [&](i64 b, bf16x8 (&xa)[MT][KS]) {
-    const i64 j0 = b * KB;
-    const i64 p = j0 / a.Q, q0 = j0 - p * a.Q;
+  struct Buf {
+    bf16x8 x[MT][KS];
+    float pre[NT];
+    int kb;
+  };
+
+  auto load = [&](i64 t, Buf& b) {
+    if (t >= T) return;
+    const int tq = (int)t / nkb;
+    const int kb = (int)t - tq * nkb;
+    const i64 slow = s_lo + wave + (i64)tq * WAVES;
+    b.kb = kb;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int r = 16 * nt + c;
+      b.pre[nt] = (a.pre && r < a.rcount) ? a.pre[slow * a.R + a.r0 + r] : 1.0f;
+    }
+    const int sl32 = (int)slow, qs32 = (int)a.qs;
+    const i64 p = sl32 / qs32;
+    const i64 q0 = (i64)(sl32 - (int)p * qs32) * a.fast_size + (i64)w * a.kw + kb * KB;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
       const uint16_t* base = X + (p * a.I + rowc[mt]) * a.Q + q0 + 8 * g;
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) xa[mt][ks] = *reinterpret_cast<const bf16x8*>(base + 32 * ks);
+      for (int ks = 0; ks < KS; ++ks) b.x[mt][ks] = *reinterpret_cast<const bf16x8*>(base + 32 * ks);
     }
   };
 
-  bf16x8 xa[MT][KS], xn[MT][KS];
-  if (b_begin < b_end) {
-    gen_kr_block<RT, KB, PITCH, NTH>(sKR[0], a, b_begin * KB, tid);
-    loadx(b_begin, xa);
-  }
-  __syncthreads();
-  for (i64 b = b_begin; b < b_end; ++b) {
-    const int buf = (int)((b - b_begin) & 1);
-    if (b + 1 < b_end) {
-      loadx(b + 1, xn);
-      gen_kr_block<RT, KB, PITCH, NTH>(sKR[buf ^ 1], a, (b + 1) * KB, tid);
-    }
-    const uint16_t* s = sKR[buf];
+  auto compute = [&](i64 t, const Buf& b) {
+    if (t >= T) return;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
-        const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&s[(16 * nt + c) * PITCH + 32 * ks + 8 * g]);
+        const int off = (16 * nt + c) * PITCH + b.kb * KB + 32 * ks + 8 * g;
+        const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&sHi[off]);
+        const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&sLo[off]);
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-          acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa[mt][ks], bv, acc[mt][nt], 0, 0, 0);
+        for (int mt = 0; mt < MT; ++mt) {
+          Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b.x[mt][ks], bh, Tacc[mt][nt], 0, 0, 0);
+          Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b.x[mt][ks], bl, Tacc[mt][nt], 0, 0, 0);
+        }
       }
-    __syncthreads();
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-      for (int ks = 0; ks < KS; ++ks) xa[mt][ks] = xn[mt][ks];
+    if (b.kb == nkb - 1) fold_slow<MT, NT>(acc, Tacc, b.pre);
+  };
+
+  Buf b0, b1, b2, b3;
+  load(0, b0);
+  load(1, b1);
+  load(2, b2);
+  __syncthreads();
+  for (i64 t = 0; t < T; t += 4) {
+    load(t + 3, b3);
+    compute(t, b0);
+    load(t + 4, b0);
+    compute(t + 1, b1);
+    load(t + 5, b1);
+    compute(t + 2, b2);
+    load(t + 6, b2);
+    compute(t + 3, b3);
   }
 
+  wg_reduce_acc<MT, NT, WAVES>(acc, smem, wave, lane);
+  if (wave != 0) return;
   float* out = partial + (i64)split * a.I * a.R;
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
-      const int col = a.r0 + 16 * nt + c;
       if (16 * nt + c >= a.rcount) continue;
+      const int col = a.r0 + 16 * nt + c;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const i64 row = row_base + mt * 16 + 4 * g + v;
+        const i64 row = row_base + 16 * mt + 4 * g + v;
         if (row < a.I) out[row * a.R + col] = acc[mt][nt][v];
       }
     }
@@ -468,16 +617,6 @@ Plan make_plan(int nd, const i64* shape, int mode) {
 
 }  // namespace
 
-// Splits so that the grid fills the chip while the partial volume stays small:
-// about 2 workgroups per CU.
-static int choose_splits(tenrso_ctx* ctx, i64 tiles, i64 nblocks) {
-  const i64 target = 2LL * ctx->num_cus;
-  i64 s = std::max<i64>(1, target / std::max<i64>(1, tiles));
-  s = std::min<i64>(s, nblocks);
-  s = std::min<i64>(s, 4096);
-  return (int)std::max<i64>(1, s);
-}
-
 int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int nd, const i64* shape,
                  const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res) {
   Plan pl = make_plan(nd, shape, mode);
@@ -541,51 +680,94 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
   a.fast = fastf;
 
   const bool caseB = pl.Q == 1;
-  int KB = bf16 ? 64 : 32;
-  if (bf16 && pl.fast_size % 64 != 0 && pl.fast_size % 32 == 0) KB = 32;
-  const bool mfma_ok = pl.n_oth >= 1 && pl.J > 0 && (pl.fast_size % KB == 0) && !(bf16 && caseB) && I > 0;
+  const int KB = bf16 ? ((pl.fast_size % 64 == 0) ? 64 : 32) : 32;
+  constexpr int KWMAX = 128;  // LDS window bound (fp32: 64 x 132 floats)
+  // LDS window of the fastest mode: the largest multiple of KB that divides it
+  int kw = 0;
+  for (i64 cand = std::min<i64>(KWMAX, pl.fast_size) / KB * KB; cand >= KB; cand -= KB)
+    if (pl.fast_size % cand == 0) {
+      kw = (int)cand;
+      break;
+    }
+  const bool mfma_ok = pl.n_oth >= 1 && pl.J > 0 && kw > 0 && !(bf16 && caseB) && I > 0;
 
   float* partial = nullptr;
   int splits = 1;
   if (mfma_ok) {
-    constexpr int WAVES = 4, MT = 1;
-    const i64 rows_per_wg = WAVES * 16 * MT;
-    const i64 tiles = (i64)cdiv(I, rows_per_wg) * rchunks;
-    a.nblocks = pl.J / KB;
-    splits = choose_splits(ctx, tiles, a.nblocks);
-    a.splits = splits;
+    constexpr int WAVES = 4;
+    const int MT = (caseB && (I % 2 != 0)) ? 1 : 2;
+    const i64 rows_per_wg = 16 * MT;
+    const i64 rtiles = (i64)cdiv(I, rows_per_wg);
+    const i64 tiles = rtiles * rchunks;
+    a.kw = kw;
+    a.nwin = (int)(pl.fast_size / kw);
+    a.n_slow = pl.J / pl.fast_size;
+    a.qs = caseB ? 1 : pl.Q / pl.fast_size;
+    // about wg_per_cu workgroups per CU; each workgroup keeps >= WAVES slow indices
+    const i64 target = (i64)(bf16 ? ctx->wg_per_cu_bf16 : ctx->wg_per_cu_f32) * ctx->num_cus;
+    i64 ssp = std::max<i64>(1, target / std::max<i64>(1, tiles) / a.nwin);
+    ssp = std::min<i64>(ssp, a.n_slow >= WAVES ? a.n_slow / WAVES : 1);
+    ssp = std::min<i64>(ssp, 65535 / a.nwin);
+    a.slow_splits = (int)std::max<i64>(1, ssp);
+    splits = a.nwin * a.slow_splits;
     void* pbuf = nullptr;
     TENRSO_TRY(ctx_buffer(ctx, SLOT_PARTIAL, (size_t)splits * nout * sizeof(float), &pbuf));
     partial = (float*)pbuf;
     const dim3 block(WAVES * 64);
-    for (int rc = 0; rc < rchunks; ++rc) {
-      a.r0 = rc * RT;
-      a.rcount = (int)std::min<i64>(RT, R - a.r0);
-      const dim3 grid(cdiv(I, rows_per_wg), splits, 1);
-      if (!bf16) {
-        const float* Xf = (const float*)Xuse;
-        if (!caseB) {
-          if (NT == 1) hipLaunchKernelGGL((mttkrp_f32_mfma<1, WAVES, MT, false>), grid, block, 0, ctx->stream, Xf, a, partial);
-          else if (NT == 2) hipLaunchKernelGGL((mttkrp_f32_mfma<2, WAVES, MT, false>), grid, block, 0, ctx->stream, Xf, a, partial);
-          else hipLaunchKernelGGL((mttkrp_f32_mfma<4, WAVES, MT, false>), grid, block, 0, ctx->stream, Xf, a, partial);
+    const dim3 grid((unsigned)rtiles, splits, 1);
+#define TENRSO_F32(NTV, MTV, CB) \
+  hipLaunchKernelGGL((mttkrp_f32_mfma<NTV, WAVES, MTV, CB, KWMAX>), grid, block, 0, ctx->stream, Xf, a, partial)
+#define TENRSO_B16(NTV, KSV) \
+  hipLaunchKernelGGL((mttkrp_bf16_mfma<NTV, WAVES, 2, KSV, KWMAX>), grid, block, 0, ctx->stream, Xb, a, partial)
+    auto launch_main = [&]() -> int {
+      for (int rc = 0; rc < rchunks; ++rc) {
+        a.r0 = rc * RT;
+        a.rcount = (int)std::min<i64>(RT, R - a.r0);
+        if (!bf16) {
+          const float* Xf = (const float*)Xuse;
+          if (!caseB) {
+            if (NT == 1) TENRSO_F32(1, 2, false);
+            else if (NT == 2) TENRSO_F32(2, 2, false);
+            else TENRSO_F32(4, 2, false);
+          } else if (MT == 2) {
+            if (NT == 1) TENRSO_F32(1, 2, true);
+            else if (NT == 2) TENRSO_F32(2, 2, true);
+            else TENRSO_F32(4, 2, true);
+          } else {
+            if (NT == 1) TENRSO_F32(1, 1, true);
+            else if (NT == 2) TENRSO_F32(2, 1, true);
+            else TENRSO_F32(4, 1, true);
+          }
         } else {
-          if (NT == 1) hipLaunchKernelGGL((mttkrp_f32_mfma<1, WAVES, MT, true>), grid, block, 0, ctx->stream, Xf, a, partial);
-          else if (NT == 2) hipLaunchKernelGGL((mttkrp_f32_mfma<2, WAVES, MT, true>), grid, block, 0, ctx->stream, Xf, a, partial);
-          else hipLaunchKernelGGL((mttkrp_f32_mfma<4, WAVES, MT, true>), grid, block, 0, ctx->stream, Xf, a, partial);
+          const uint16_t* Xb = (const uint16_t*)Xuse;
+          if (KB == 64) {
+            if (NT == 1) TENRSO_B16(1, 2);
+            else if (NT == 2) TENRSO_B16(2, 2);
+            else TENRSO_B16(4, 2);
+          } else {
+            if (NT == 1) TENRSO_B16(1, 1);
+            else if (NT == 2) TENRSO_B16(2, 1);
+            else TENRSO_B16(4, 1);
+          }
         }
-      } else {
-        const uint16_t* Xb = (const uint16_t*)Xuse;
-        if (KB == 64) {
-          if (NT == 1) hipLaunchKernelGGL((mttkrp_bf16_mfma<1, WAVES, MT, 2>), grid, block, 0, ctx->stream, Xb, a, partial);
-          else if (NT == 2) hipLaunchKernelGGL((mttkrp_bf16_mfma<2, WAVES, MT, 2>), grid, block, 0, ctx->stream, Xb, a, partial);
-          else hipLaunchKernelGGL((mttkrp_bf16_mfma<4, WAVES, MT, 2>), grid, block, 0, ctx->stream, Xb, a, partial);
-        } else {
-          if (NT == 1) hipLaunchKernelGGL((mttkrp_bf16_mfma<1, WAVES, MT, 1>), grid, block, 0, ctx->stream, Xb, a, partial);
-          else if (NT == 2) hipLaunchKernelGGL((mttkrp_bf16_mfma<2, WAVES, MT, 1>), grid, block, 0, ctx->stream, Xb, a, partial);
-          else hipLaunchKernelGGL((mttkrp_bf16_mfma<4, WAVES, MT, 1>), grid, block, 0, ctx->stream, Xb, a, partial);
-        }
+        TENRSO_HIP_TRY(hipGetLastError());
       }
-      TENRSO_HIP_TRY(hipGetLastError());
+      return TENRSO_OK;
+    };
+#undef TENRSO_F32
+#undef TENRSO_B16
+    if (ctx->profile_reps > 0) {
+      // profiling: the main kernel alone, `profile_reps` times between two events
+      TENRSO_TRY(launch_main());
+      TENRSO_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+      for (int rep = 0; rep < ctx->profile_reps; ++rep) TENRSO_TRY(launch_main());
+      TENRSO_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+      TENRSO_HIP_TRY(hipEventSynchronize(ctx->ev1));
+      float ms = 0.f;
+      TENRSO_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+      ctx->last_kernel_ms = (double)ms / ctx->profile_reps;
+    } else {
+      TENRSO_TRY(launch_main());
     }
   } else {
     GenericArgs ga{};

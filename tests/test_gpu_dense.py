@@ -94,23 +94,36 @@ def test_mttkrp_c2_shape_fp32():
         assert e <= 1e-5
 
 
-def test_mttkrp_bf16_error_bound():
-    print("[case] bf16 mttkrp 128x128x64 rank 32", flush=True)
+def bf16_round(a):
+    """Round float32 -> bfloat16 (RNE) -> float32 in numpy."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+@pytest.mark.parametrize("shape,rank", [((128, 128, 64), 32), ((64, 96, 128), 64), ((32, 32, 32, 32), 16)])
+def test_mttkrp_bf16_error_bound(shape, rank):
+    print(f"[case] bf16 mttkrp {shape} rank {rank}", flush=True)
     rng = np.random.default_rng(1)
-    X = rng.standard_normal((128, 128, 64)).astype(np.float32)
-    F = [rng.standard_normal((n, 32)).astype(np.float32) for n in X.shape]
+    X = rng.standard_normal(shape).astype(np.float32)
+    F = [rng.standard_normal((n, rank)).astype(np.float32) for n in X.shape]
     Xd = dev(X)
     Xb = kernels.to_bf16(Xd)
-    # conversion is RNE, identical to torch's
+    # conversion is RNE, identical to torch's and to the numpy rounding
     assert torch.equal(Xb.view(torch.int16), Xd.to(torch.bfloat16).view(torch.int16))
+    assert np.array_equal(Xb.float().cpu().numpy(), bf16_round(X))
     Fd = [dev(f) for f in F]
-    for mode in range(3):
+    Xr64 = bf16_round(X).astype(np.float64)
+    for mode in range(len(shape)):
         got = kernels.mttkrp_bf16(Xb, Fd, mode).cpu().numpy()
         ref = oracle.mttkrp_fast(X.astype(np.float64), f64(F), mode)
-        # exact MTTKRP of the bf16-rounded tensor and bf16-rounded KR rows, FP64
-        e = rel(got, ref)
-        print(f"  mode {mode}: rel frob err vs fp64 {e:.3e}", flush=True)
+        # the factors enter as an exact bf16 hi+lo pair, so the kernel computes the
+        # MTTKRP of the bf16-rounded tensor to FP32-accumulation accuracy
+        ref_b = oracle.mttkrp_fast(Xr64, f64(F), mode)
+        e, eb = rel(got, ref), rel(got, ref_b)
+        print(f"  mode {mode}: rel frob err vs fp64 {e:.3e}, vs fp64 of bf16(X) {eb:.3e}", flush=True)
         assert e < 5e-3
+        assert eb < 2e-5
 
 
 def test_khatri_rao_bit_exact():
