@@ -189,6 +189,38 @@ int tenrso_cp_als_f32_host(tenrso_ctx* ctx, const float* X, int ndim, const int6
                            float* const* factors, const tenrso_cp_options* opts, tenrso_cp_result* result,
                            double* fit_history);
 
+/* ------------------------------------------------------------------------
+ * Sparse tensors.  A COO tensor (tenrso-sparse/src/coo.rs:73-348) is uploaded
+ * once and sorted on the device, per output mode, in the lexicographic order of
+ * CsfTensor::from_coo with mode_order [mode, others ascending] (csf.rs:137-264).
+ * Indices are int32 SoA: indices[k * nnz + e] is the mode-k index of nonzero e.
+ * Duplicate coordinates are summed (the dense form of CooTensor::deduplicate,
+ * coo.rs:320-348).  Validation follows CooTensor::new (coo.rs:101-149).
+ * ------------------------------------------------------------------------ */
+typedef struct tenrso_sptensor tenrso_sptensor;
+
+int tenrso_sp_create(tenrso_ctx* ctx, int ndim, const int64_t* shape, int64_t nnz, const int32_t* indices,
+                     const float* values, int host_pointers, tenrso_sptensor** out);
+int tenrso_sp_destroy(tenrso_sptensor* t);
+/* Sorted per-mode arrays, copied to host buffers: rowptr (shape[mode]+1),
+ * row (nnz), others ((ndim-1) x nnz, modes ascending), values (nnz).  Any
+ * pointer may be NULL to skip it. */
+int tenrso_sp_mode_arrays(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, int64_t* rowptr, int32_t* row,
+                          int32_t* others, float* values);
+/* CSF of mode order [mode, others ascending] with standard fiber pointers
+ * (fptr[l][f]..fptr[l][f+1] index level l+1; the leaf level has one entry per
+ * nonzero), unlike the nnz-offset pointers of csf.rs:209,246.  Call with
+ * fptr/fids NULL to get nfib[l] (fibers per level, ndim entries). */
+int tenrso_sp_csf(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, int64_t* nfib, int64_t* const* fptr,
+                  int32_t* const* fids, float* values);
+/* Sparse MTTKRP (no reference counterpart; contract of the dense mttkrp on the
+ * tensor the COO represents).  factors / out are device pointers. */
+int tenrso_sp_mttkrp_f32(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* factors, int64_t rank,
+                         int mode, float* out);
+/* CP-ALS on a sparse tensor: the cp_als rules above with the sparse MTTKRP. */
+int tenrso_sp_cp_als_f32(tenrso_ctx* ctx, const tenrso_sptensor* t, float* const* factors,
+                         const tenrso_cp_options* opts, tenrso_cp_result* result, double* fit_history);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,12 @@ SIGNATURES = {
     "tenrso_cp_reconstruct_f32": ([_vp, _i, _vp, _vp, _i64, _vp, _vp], _i),
     "tenrso_cp_als_f32": ([_vp, _vp, _i, _vp, _vp, C.POINTER(CpOptions), C.POINTER(CpResult), _vp], _i),
     "tenrso_cp_als_f32_host": ([_vp, _vp, _i, _vp, _vp, C.POINTER(CpOptions), C.POINTER(CpResult), _vp], _i),
+    "tenrso_sp_create": ([_vp, _i, _vp, _i64, _vp, _vp, _i, C.POINTER(_vp)], _i),
+    "tenrso_sp_destroy": ([_vp], _i),
+    "tenrso_sp_mode_arrays": ([_vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
+    "tenrso_sp_csf": ([_vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
+    "tenrso_sp_mttkrp_f32": ([_vp, _vp, _vp, _i64, _i, _vp], _i),
+    "tenrso_sp_cp_als_f32": ([_vp, _vp, _vp, C.POINTER(CpOptions), C.POINTER(CpResult), _vp], _i),
 }
 
 _lib = None

@@ -9,6 +9,8 @@
 #include "cp_als.hpp"
 #include "kr_reconstruct.hpp"
 #include "mttkrp.hpp"
+#include "cp_linalg.hpp"
+#include "sparse.hpp"
 
 using namespace tenrso;
 
@@ -337,4 +339,106 @@ int tenrso_cp_als_f32_host(tenrso_ctx* ctx, const float* X, int ndim, const int6
   return TENRSO_OK;
 }
 
+// ---------------------------------------------------------------- sparse
+int tenrso_sp_create(tenrso_ctx* ctx, int ndim, const int64_t* shape, int64_t nnz, const int32_t* indices,
+                     const float* values, int host_pointers, tenrso_sptensor** out) {
+  clear_error();
+  if (bad_ctx(ctx) || !out || !shape) {
+    if (ctx) set_error("null argument");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  TENRSO_TRY(set_device(ctx));
+  return sp_create(ctx, ndim, shape, nnz, indices, values, host_pointers != 0, out);
+}
+
+int tenrso_sp_destroy(tenrso_sptensor* t) {
+  sp_destroy(t);
+  return TENRSO_OK;
+}
+
+int tenrso_sp_mode_arrays(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, int64_t* rowptr, int32_t* row,
+                          int32_t* others, float* values) {
+  clear_error();
+  if (bad_ctx(ctx) || !t) return TENRSO_ERR_INVALID_ARGUMENT;
+  if (mode < 0 || mode >= t->nd) {
+    set_error("Mode %d out of bounds for tensor with rank %d", mode, t->nd);
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  TENRSO_TRY(set_device(ctx));
+  const SpMode& md = t->modes[mode];
+  const i64 nnz = t->nnz;
+  if (rowptr)
+    TENRSO_HIP_TRY(hipMemcpyAsync(rowptr, md.rowptr, (size_t)(md.rows + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (nnz > 0) {
+    if (row) TENRSO_HIP_TRY(hipMemcpyAsync(row, md.row, (size_t)nnz * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (others && t->nd > 1)
+      TENRSO_HIP_TRY(hipMemcpyAsync(others, md.oth, (size_t)nnz * (t->nd - 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (values) TENRSO_HIP_TRY(hipMemcpyAsync(values, md.vals, (size_t)nnz * 4, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TENRSO_OK;
+}
+
+int tenrso_sp_csf(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, int64_t* nfib, int64_t* const* fptr,
+                  int32_t* const* fids, float* values) {
+  clear_error();
+  if (bad_ctx(ctx) || !t || !nfib) return TENRSO_ERR_INVALID_ARGUMENT;
+  if (mode < 0 || mode >= t->nd) {
+    set_error("Mode %d out of bounds for tensor with rank %d", mode, t->nd);
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  TENRSO_TRY(set_device(ctx));
+  std::vector<std::vector<int64_t>> fp;
+  std::vector<std::vector<int32_t>> fi;
+  std::vector<float> vals;
+  TENRSO_TRY(sp_export_csf(ctx, t, mode, fp, fi, vals));
+  for (int l = 0; l < t->nd; ++l) nfib[l] = (int64_t)fi[l].size();
+  if (fptr && fids) {
+    for (int l = 0; l < t->nd; ++l) {
+      if (fptr[l]) std::memcpy(fptr[l], fp[l].data(), fp[l].size() * sizeof(int64_t));
+      if (fids[l]) std::memcpy(fids[l], fi[l].data(), fi[l].size() * sizeof(int32_t));
+    }
+  }
+  if (values) std::memcpy(values, vals.data(), vals.size() * sizeof(float));
+  return TENRSO_OK;
+}
+
+int tenrso_sp_mttkrp_f32(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* factors, int64_t rank,
+                         int mode, float* out) {
+  clear_error();
+  if (bad_ctx(ctx) || !t) return TENRSO_ERR_INVALID_ARGUMENT;
+  TENRSO_TRY(check_mttkrp_basic(t->nd, t->shape, rank, mode));
+  TENRSO_TRY(set_device(ctx));
+  TENRSO_TRY(timed_begin(ctx));
+  TENRSO_TRY(sp_mttkrp(ctx, t, factors, rank, mode, out));
+  return timed_end(ctx);
+}
+
+int tenrso_sp_cp_als_f32(tenrso_ctx* ctx, const tenrso_sptensor* t, float* const* factors,
+                         const tenrso_cp_options* opts, tenrso_cp_result* result, double* fit_history) {
+  clear_error();
+  if (bad_ctx(ctx) || !t || !opts) return TENRSO_ERR_INVALID_ARGUMENT;
+  if (opts->precision != TENRSO_PREC_FP32) {
+    set_error("sparse CP-ALS runs in FP32");
+    return TENRSO_ERR_UNSUPPORTED;
+  }
+  const i64 R = opts->rank;
+  AlsProblem pb;
+  pb.nd = t->nd;
+  pb.shape = t->shape;
+  pb.norm_min = [&](double* tmp, double* scal) { return tensor_norm_min(ctx, t->vals, t->nnz, tmp, scal); };
+  pb.mttkrp = [&](int mode, const float* const* F, MttkrpResult* mr) -> int {
+    i64 maxI = 0;
+    for (int k = 0; k < t->nd; ++k) maxI = std::max<i64>(maxI, t->shape[k]);
+    void* pm = nullptr;
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_SPM, (size_t)maxI * R * sizeof(float), &pm));
+    TENRSO_TRY(sp_mttkrp(ctx, t, F, R, mode, (float*)pm));
+    mr->partial = (const float*)pm;
+    mr->splits = 1;
+    return TENRSO_OK;
+  };
+  return cp_als_run(ctx, pb, factors, opts, result, fit_history);
+}
+
 }  // extern "C"
+

@@ -62,6 +62,7 @@ struct tenrso_ctx {
   int profile_reps = 0;               // > 0: mttkrp_dense times its main kernel
   int wg_per_cu_f32 = 2;              // MTTKRP grid sizing (TENRSO_WG_PER_CU_F32 / _BF16)
   int wg_per_cu_bf16 = 2;
+  bool serial_solve = false;          // TENRSO_SERIAL_SOLVE=1: no side stream (profiling aid)
   std::vector<tenrso::DevBuf> bufs;  // indexed by slot id
   void* pinned = nullptr;            // small pinned host buffer for scalar readback
 };
@@ -82,6 +83,8 @@ enum BufSlot : int {
   SLOT_STAGE_OUT,
   SLOT_TMP,
   SLOT_TMP2,
+  SLOT_IP,
+  SLOT_SPM,
   SLOT_COUNT
 };
 

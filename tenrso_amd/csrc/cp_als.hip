@@ -23,8 +23,10 @@
 
 namespace tenrso {
 
-int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, float* const* F,
-                  const tenrso_cp_options* o, tenrso_cp_result* res, double* hist) {
+int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp_options* o, tenrso_cp_result* res,
+               double* hist) {
+  const int nd = pb.nd;
+  const i64* shape = pb.shape;
   const i64 R = o->rank;
   // validation order of cp/core.rs:82-95, 250-275
   if (R == 0) {
@@ -60,34 +62,25 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
     set_error("orthogonal constraint is not supported by the device CP-ALS");
     return TENRSO_ERR_UNSUPPORTED;
   }
-  const bool bf16 = o->precision == TENRSO_PREC_BF16;
-  if (o->precision != TENRSO_PREC_FP32 && !bf16) {
-    set_error("unknown precision %d", o->precision);
-    return TENRSO_ERR_INVALID_ARGUMENT;
-  }
   TENRSO_TRY(set_device(ctx));
-  i64 total = 1, maxI = 0;
-  for (int k = 0; k < nd; ++k) {
-    total *= shape[k];
-    maxI = std::max(maxI, shape[k]);
-  }
+  i64 maxI = 0;
+  for (int k = 0; k < nd; ++k) maxI = std::max(maxI, shape[k]);
 
   // ---- device workspace
-  void *p_scal, *p_G, *p_Winv, *p_M, *p_ip, *p_tmp;
+  void *p_scal, *p_G, *p_Winv, *p_M, *p_tmp;
   TENRSO_TRY(ctx_buffer(ctx, SLOT_SCALARS, SC_COUNT * sizeof(double), &p_scal));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_GRAMS, (size_t)(nd + 1) * R * R * sizeof(double), &p_G));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_SOLVE, (size_t)SOLVE_RMAX * SOLVE_RMAX * sizeof(double), &p_Winv));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_M64, (size_t)maxI * R * sizeof(double), &p_M));
-  TENRSO_TRY(ctx_buffer(ctx, SLOT_TMP2, (size_t)std::max<i64>(4096, maxI + 16) * sizeof(double), &p_tmp));
-  p_ip = p_tmp;
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_IP, (size_t)std::max<i64>(4096, maxI + 16) * sizeof(double), &p_tmp));
   double* scal = (double*)p_scal;
   double* G = (double*)p_G;
   double* Winv = (double*)p_Winv;
   double* M = (double*)p_M;
-  double* ip_part = (double*)p_ip;
+  double* ip_part = (double*)p_tmp;
 
   TENRSO_TRY(fill_f64(ctx, scal, SC_COUNT, 0.0));
-  TENRSO_TRY(tensor_norm_min(ctx, X, total, ip_part, scal));
+  TENRSO_TRY(pb.norm_min(ip_part, scal));
 
   const bool nonneg = o->nonnegative != 0;
   if (nonneg && o->validate_nonneg_input) {
@@ -101,18 +94,7 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
   }
   if (nonneg)
     for (int k = 0; k < nd; ++k) TENRSO_TRY(clamp_nonneg(ctx, F[k], shape[k] * R));  // core.rs:281-285
-
-  const void* Xk = X;
-  const void* Xt = nullptr;
-  if (bf16) {
-    void *pb, *pt;
-    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16, (size_t)total * 2, &pb));
-    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * 2, &pt));
-    TENRSO_TRY(convert_bf16(ctx, X, nd, shape, false, (uint16_t*)pb));
-    TENRSO_TRY(convert_bf16(ctx, X, nd, shape, true, (uint16_t*)pt));
-    Xk = pb;
-    Xt = pt;
-  }
+  if (pb.prepare) TENRSO_TRY(pb.prepare());
 
   for (int k = 0; k < nd; ++k) TENRSO_TRY(gram_f64(ctx, F[k], shape[k], R, G + (i64)k * R * R));
 
@@ -140,23 +122,27 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
     int nblk = 0;
     for (int mode = 0; mode < nd; ++mode) {
       const i64 I = shape[mode];
-      // Gamma_n^{-1} only needs the Grams, so it runs on the side stream while
-      // the MTTKRP of the same mode runs on the main stream.
+      // Side stream: the Gram of the factor updated just before (mode - 1) and
+      // Gamma_n^{-1}; both only need factors that are final for this mode, so
+      // they run while the MTTKRP of mode n runs on the main stream.
       TENRSO_HIP_TRY(hipEventRecord(ctx->ev_g, ctx->stream));
-      TENRSO_HIP_TRY(hipStreamWaitEvent(ctx->aux, ctx->ev_g, 0));
-      TENRSO_TRY(normal_eq_inverse(ctx, G, nd, mode, R, reg, Winv, scal, ctx->aux));
-      TENRSO_HIP_TRY(hipEventRecord(ctx->ev_w, ctx->aux));
+      hipStream_t side = ctx->serial_solve ? ctx->stream : ctx->aux;
+      if (!ctx->serial_solve) TENRSO_HIP_TRY(hipStreamWaitEvent(ctx->aux, ctx->ev_g, 0));
+      if (mode > 0) TENRSO_TRY(gram_f64(ctx, F[mode - 1], shape[mode - 1], R, G + (i64)(mode - 1) * R * R, side));
+      TENRSO_TRY(normal_eq_inverse(ctx, G, nd, mode, R, reg, Winv, scal, side));
+      TENRSO_HIP_TRY(hipEventRecord(ctx->ev_w, side));
       MttkrpResult mr;
-      TENRSO_TRY(mttkrp_dense(ctx, bf16 ? TENRSO_PREC_BF16 : TENRSO_PREC_FP32, Xk, Xt, nd, shape, Fc, R, mode,
-                              MttkrpOut::PARTIAL, nullptr, &mr));
-      TENRSO_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_w, 0));
+      TENRSO_TRY(pb.mttkrp(mode, Fc, &mr));
+      if (!ctx->serial_solve) TENRSO_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_w, 0));
       TENRSO_TRY(solve_rows(ctx, mr.partial, mr.splits, nullptr, I, R, Winv, epi, F[mode], ip_part, &nblk));
-      TENRSO_TRY(gram_f64(ctx, F[mode], I, R, G + (i64)mode * R * R));
     }
+    // Gram of the last factor (needed by the fit and by the next iteration)
+    TENRSO_TRY(gram_f64(ctx, F[nd - 1], shape[nd - 1], R, G + (i64)(nd - 1) * R * R));
     if (o->fit_from_mode0) {
       // cp/helpers.rs:558-579: <X, Xhat> from a mode-0 MTTKRP with every updated factor
-      TENRSO_TRY(mttkrp_dense(ctx, bf16 ? TENRSO_PREC_BF16 : TENRSO_PREC_FP32, Xk, Xt, nd, shape, Fc, R, 0,
-                              MttkrpOut::F64, M, nullptr));
+      MttkrpResult mr;
+      TENRSO_TRY(pb.mttkrp(0, Fc, &mr));
+      TENRSO_TRY(reduce_partials(ctx, mr.partial, mr.splits, shape[0] * R, nullptr, M));
       TENRSO_TRY(inner_rows(ctx, M, F[0], shape[0], R, ip_part, &nblk));
     }
     TENRSO_TRY(fit_from_grams(ctx, G, nd, R, ip_part, nblk, scal));
@@ -196,6 +182,40 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
     res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   return TENRSO_OK;
+}
+
+int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, float* const* F,
+                  const tenrso_cp_options* o, tenrso_cp_result* res, double* hist) {
+  const bool bf16 = o->precision == TENRSO_PREC_BF16;
+  if (o->precision != TENRSO_PREC_FP32 && !bf16) {
+    set_error("unknown precision %d", o->precision);
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  i64 total = 1;
+  for (int k = 0; k < nd; ++k) total *= shape[k];
+  const void* Xk = X;
+  const void* Xt = nullptr;
+  const i64 R = o->rank;
+  AlsProblem pb;
+  pb.nd = nd;
+  pb.shape = shape;
+  pb.norm_min = [&](double* tmp, double* scal) { return tensor_norm_min(ctx, X, total, tmp, scal); };
+  pb.prepare = [&]() -> int {
+    if (!bf16) return TENRSO_OK;
+    void *pb16, *pt;
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16, (size_t)total * 2, &pb16));
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * 2, &pt));
+    TENRSO_TRY(convert_bf16(ctx, X, nd, shape, false, (uint16_t*)pb16));
+    TENRSO_TRY(convert_bf16(ctx, X, nd, shape, true, (uint16_t*)pt));
+    Xk = pb16;
+    Xt = pt;
+    return TENRSO_OK;
+  };
+  pb.mttkrp = [&](int mode, const float* const* Fc, MttkrpResult* mr) {
+    return mttkrp_dense(ctx, bf16 ? TENRSO_PREC_BF16 : TENRSO_PREC_FP32, Xk, Xt, nd, shape, Fc, R, mode,
+                        MttkrpOut::PARTIAL, nullptr, mr);
+  };
+  return cp_als_run(ctx, pb, F, o, res, hist);
 }
 
 }  // namespace tenrso

@@ -1,9 +1,25 @@
 // cp_als.hpp -- device CP-ALS driver.
 #pragma once
+#include <functional>
+
 #include "common.hpp"
+#include "mttkrp.hpp"
 
 namespace tenrso {
 
+// What the ALS loop needs from a tensor format: ||X||^2 and min(X) into the
+// scalar block, an optional one-time preparation, and the mode-n MTTKRP as fp32
+// split partials [splits][I_n][R] (summed in FP64 by the row solve).
+struct AlsProblem {
+  int nd = 0;
+  const i64* shape = nullptr;
+  std::function<int(double* tmp, double* scal)> norm_min;
+  std::function<int()> prepare;
+  std::function<int(int mode, const float* const* F, MttkrpResult* out)> mttkrp;
+};
+
+int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp_options* o, tenrso_cp_result* res,
+               double* hist);
 int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, float* const* F,
                   const tenrso_cp_options* o, tenrso_cp_result* res, double* hist);
 

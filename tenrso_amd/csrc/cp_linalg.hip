@@ -88,49 +88,42 @@ int tensor_norm_min(tenrso_ctx* ctx, const float* X, i64 n, double* tmp, double*
 }
 
 // ---------------------------------------------------------------- Gram A^T A
-// grid: ceil(R / 4) workgroups; workgroup b computes Gram rows 4b .. 4b+3 (all
-// columns).  Rows of A are staged through LDS in chunks of GRAM_CH rows with
-// 16-byte loads; each thread accumulates one (r, c) entry in FP64 over the
-// chunk order (deterministic).  R <= SOLVE_RMAX, R % 4 == 0 handled by masking.
-constexpr int GRAM_CH = 128;
-
-__global__ __launch_bounds__(256) void gram_rows_kernel(const float* __restrict__ A, i64 rows, int R,
-                                                        double* __restrict__ G) {
-  __shared__ __attribute__((aligned(16))) float sA[GRAM_CH * SOLVE_RMAX];
-  const int c = threadIdx.x & 63, rr = threadIdx.x >> 6;
-  const int r = blockIdx.x * 4 + rr;
+// grid: R workgroups of 1024 threads; workgroup r computes Gram row r.  Thread
+// (ch, c) sums A[i][r] A[i][c] in FP64 over row chunk ch (16 chunks), with all of
+// its loads independent; the 16 chunk sums are added in fixed order.
+__global__ __launch_bounds__(1024) void gram_rows_kernel(const float* __restrict__ A, i64 rows, int R,
+                                                         double* __restrict__ G) {
+  __shared__ double part[16 * SOLVE_RMAX];
+  const int c = threadIdx.x & 63, ch = threadIdx.x >> 6;
+  const int r = blockIdx.x;
   double acc0 = 0.0, acc1 = 0.0;
-  for (i64 i0 = 0; i0 < rows; i0 += GRAM_CH) {
-    const int n = (int)((rows - i0) < GRAM_CH ? (rows - i0) : GRAM_CH);
-    const float* src = A + i0 * R;
-    const int tot = n * R;
-    __syncthreads();
-    if ((R & 3) == 0) {
-      for (int e = threadIdx.x * 4; e < tot; e += 1024)
-        *reinterpret_cast<float4*>(&sA[e]) = *reinterpret_cast<const float4*>(&src[e]);
-    } else {
-      for (int e = threadIdx.x; e < tot; e += 256) sA[e] = src[e];
+  if (c < R) {
+    const i64 i0 = (i64)ch * rows / 16, i1 = (i64)(ch + 1) * rows / 16;
+    i64 i = i0;
+    for (; i + 2 <= i1; i += 2) {
+      acc0 += (double)A[i * R + r] * (double)A[i * R + c];
+      acc1 += (double)A[(i + 1) * R + r] * (double)A[(i + 1) * R + c];
     }
-    __syncthreads();
-    if (r < R && c < R) {
-      int i = 0;
-      for (; i + 2 <= n; i += 2) {
-        acc0 += (double)sA[i * R + r] * (double)sA[i * R + c];
-        acc1 += (double)sA[(i + 1) * R + r] * (double)sA[(i + 1) * R + c];
-      }
-      if (i < n) acc0 += (double)sA[i * R + r] * (double)sA[i * R + c];
-    }
+    if (i < i1) acc0 += (double)A[i * R + r] * (double)A[i * R + c];
   }
-  if (r < R && c < R) G[(i64)r * R + c] = acc0 + acc1;
+  part[ch * SOLVE_RMAX + c] = acc0 + acc1;
+  __syncthreads();
+  if (ch == 0 && c < R) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) s += part[k * SOLVE_RMAX + c];
+    G[(i64)r * R + c] = s;
+  }
 }
 
-int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G) {
+int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G, hipStream_t stream) {
   if (R > SOLVE_RMAX) {
     set_error("rank %lld exceeds the on-device Gram bound %d", (long long)R, SOLVE_RMAX);
     return TENRSO_ERR_UNSUPPORTED;
   }
   if (R == 0) return TENRSO_OK;
-  hipLaunchKernelGGL(gram_rows_kernel, dim3(cdiv(R, 4)), dim3(256), 0, ctx->stream, A, rows, (int)R, G);
+  hipLaunchKernelGGL(gram_rows_kernel, dim3((unsigned)R), dim3(1024), 0, stream ? stream : ctx->stream, A, rows,
+                     (int)R, G);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -198,16 +191,24 @@ __device__ __forceinline__ double recip(double p) {
   return r;
 }
 
-// In-place Gauss-Jordan inverse of the 64 x 64 matrix held as av[m] = A[rg+4m][c].
-// Step j: the wave owning row j (rg == j % 4) reads the pivot with readlane,
-// publishes 1/p and the scaled pivot row; the four threads with c == j publish
-// the pivot column; one barrier; every thread updates its 16 entries.  A bad
-// pivot (<= 0 or not finite) clears `ok` instead of branching, so every thread
-// runs the same 64 steps.
+// In-place Gauss-Jordan inverse of the 64 x 64 matrix held as av[m] = A[rg+4m][c]
+// (classic in-place form: after step j, column j holds column j of the inverse).
+// Step j:
+//   * the wave owning row j (rg == j % 4) reads the pivot p = A[j][j] from lane j
+//     (readlane), publishes 1/p and the scaled pivot row prow[c] = A[j][c] / p;
+//   * the four threads with c == j publish the pivot column pcol[i] = A[i][j],
+//     except pcol[j] = p - 1, and reset their own entries to the identity
+//     column (1 on row j, 0 elsewhere);
+//   * one barrier; then every entry does a single FMA
+//       A[i][c] -= pcol[i] * prow[c]
+//     which yields A[i][c] - A[i][j] A[j][c] / p off the pivot row and column,
+//     A[j][c] / p on the pivot row, -A[i][j] / p on the pivot column and 1/p on
+//     the pivot.  A bad pivot (<= 0 or not finite) clears `ok` (no branch).
 __device__ bool gj_invert(double (&av)[16], int c, int rg, double* prow, double* pcol, double* pinv) {
   bool ok = true;
   for (int j = 0; j < NE; ++j) {
     const int b = j & 1;
+    double p = 0.0;
     if (rg == (j & 3)) {
       // pv = av[j / 4] without a dynamic register index
       double pv = 0.0;
@@ -215,29 +216,24 @@ __device__ bool gj_invert(double (&av)[16], int c, int rg, double* prow, double*
       for (int m = 0; m < 16; ++m) pv = (m == (j >> 2)) ? av[m] : pv;
       const int lo = __builtin_amdgcn_readlane(__double2loint(pv), j);
       const int hi = __builtin_amdgcn_readlane(__double2hiint(pv), j);
-      const double p = __hiloint2double(hi, lo);
+      p = __hiloint2double(hi, lo);
       const double ip = recip(p);
-      prow[b * NE + c] = pv * ip;
+      prow[b * NE + c] = (c == j) ? ip : pv * ip;
       if (c == 0) pinv[b] = (p > 0.0 && isfinite(p)) ? ip : __longlong_as_double(0x7ff8000000000000LL);
     }
     if (c == j) {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) pcol[b * NE + rg + 4 * m] = av[m];
+      for (int m = 0; m < 16; ++m) {
+        const bool diag = (rg + 4 * m) == j;
+        pcol[b * NE + rg + 4 * m] = diag ? av[m] - 1.0 : av[m];
+        av[m] = diag ? 1.0 : 0.0;
+      }
     }
     __syncthreads();
-    const double ip = pinv[b];
-    ok = ok && !isnan(ip);
+    ok = ok && !isnan(pinv[b]);
     const double ajc = prow[b * NE + c];
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int i = rg + 4 * m;
-      const double aij = pcol[b * NE + i];
-      const double upd = fma(-aij, ajc, av[m]);
-      const double colv = -aij * ip;
-      double v = (c == j) ? colv : upd;
-      if (i == j) v = (c == j) ? ip : ajc;
-      av[m] = v;
-    }
+    for (int m = 0; m < 16; ++m) av[m] = fma(-pcol[b * NE + rg + 4 * m], ajc, av[m]);
   }
   __syncthreads();
   return ok;

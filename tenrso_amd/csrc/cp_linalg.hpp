@@ -32,8 +32,8 @@ enum Scalars : int {
 
 // ||X||^2 (FP64) and min(X) into scal[SC_NORM_X], scal[SC_MIN_X].  tmp: >= 2*1024 doubles.
 int tensor_norm_min(tenrso_ctx* ctx, const float* X, i64 n, double* tmp, double* scal);
-// G = A^T A (FP64) for an fp32 rows x R factor.
-int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G);
+// G = A^T A (FP64) for an fp32 rows x R factor, on `stream` (null: the context stream).
+int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G, hipStream_t stream = nullptr);
 // Winv = (hadamard_{k != skip} G_k + reg)^{-1}, 64 x 64 (stride 64, identity padding);
 // G: [nd][R][R].  Writes the path taken into scal[SC_STATUS] (max with the previous
 // value).  Runs on `stream` (null: the context stream).

@@ -1,0 +1,509 @@
+// sparse.hip -- sparse COO tensors and sparse MTTKRP on gfx950.
+//
+// The reference has COO / CSF / HiCOO formats (tenrso-sparse/src/coo.rs:73-348,
+// csf.rs:137-264, hicoo.rs:121-205) but no sparse MTTKRP.  The contract here is
+// the dense MTTKRP (tenrso-kernels/src/mttkrp.rs:74-146) of the tensor the COO
+// represents, with duplicate coordinates summed (the dense form of
+// CooTensor::deduplicate, coo.rs:320-348):
+//   M[i_n, r] = sum_{nonzeros e} x_e * prod_{k != n} A_k[i_k(e), r].
+//
+// Layout (per output mode n, built once on the device):
+//   the nonzeros sorted lexicographically by (i_n, other modes ascending) -- the
+//   order CsfTensor::from_coo produces for mode_order [n, others...] -- stored as
+//   SoA: row (i_n, int32), the other modes' indices (int32) and values (f32),
+//   plus rowptr (int64, I_n + 1) and the CSF fiber structure (standard fptr
+//   semantics: fptr[l][f] indexes level l+1, unlike csf.rs:209,246).
+//
+// MTTKRP kernel: a worker of LPN lanes (one lane per rank column) owns a
+// contiguous chunk of SP_CH sorted nonzeros and accumulates row by row in
+// registers.  Rows strictly inside the chunk belong to the worker alone and are
+// stored directly; the chunk's first and last rows may be shared with
+// neighbouring chunks and go to two carry slots.  The carry slots (sorted by
+// row) are folded by the same tile-wise segmented reduction, level by level,
+// until one tile remains.  No atomics: the result is deterministic.
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "sparse.hpp"
+
+namespace tenrso {
+
+static inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------- build
+struct KeyArgs {
+  int nd;
+  int mode;
+  i64 nnz;
+  i64 shape[MAXD_SP];
+  const int32_t* idx;  // SoA: idx[k * nnz + e]
+};
+
+// key = (i_n, i_o1, i_o2, ...) linearised, others in ascending mode order
+__global__ __launch_bounds__(256) void make_keys_kernel(KeyArgs a, unsigned long long* __restrict__ keys,
+                                                        int32_t* __restrict__ pos) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= a.nnz) return;
+  unsigned long long k = (unsigned long long)a.idx[(i64)a.mode * a.nnz + e];
+  for (int m = 0; m < a.nd; ++m) {
+    if (m == a.mode) continue;
+    k = k * (unsigned long long)a.shape[m] + (unsigned long long)a.idx[(i64)m * a.nnz + e];
+  }
+  keys[e] = k;
+  pos[e] = (int32_t)e;
+}
+
+struct GatherArgs {
+  int nd, mode;
+  i64 nnz;
+  const int32_t* idx;
+  const float* vals;
+  const int32_t* perm;
+  int32_t* row;
+  int32_t* oth;  // (nd-1) x nnz
+  float* vout;
+};
+
+__global__ __launch_bounds__(256) void gather_sorted_kernel(GatherArgs a) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= a.nnz) return;
+  const i64 src = a.perm[e];
+  a.row[e] = a.idx[(i64)a.mode * a.nnz + src];
+  int o = 0;
+  for (int m = 0; m < a.nd; ++m) {
+    if (m == a.mode) continue;
+    a.oth[(i64)o * a.nnz + e] = a.idx[(i64)m * a.nnz + src];
+    ++o;
+  }
+  a.vout[e] = a.vals[src];
+}
+
+// rowptr[i] = first position with row >= i (binary search over the sorted rows)
+__global__ __launch_bounds__(256) void rowptr_kernel(const int32_t* __restrict__ row, i64 nnz, i64 rows,
+                                                     int64_t* __restrict__ rowptr) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i > rows) return;
+  i64 lo = 0, hi = nnz;
+  while (lo < hi) {
+    const i64 mid = (lo + hi) >> 1;
+    if ((i64)row[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  rowptr[i] = lo;
+}
+
+__global__ __launch_bounds__(256) void bounds_kernel(KeyArgs a, int* __restrict__ bad) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= a.nnz) return;
+  for (int m = 0; m < a.nd; ++m) {
+    const int32_t v = a.idx[(i64)m * a.nnz + e];
+    if (v < 0 || (i64)v >= a.shape[m]) bad[0] = 1;
+  }
+}
+
+int sp_build_mode(tenrso_ctx* ctx, tenrso_sptensor* t, int mode) {
+  SpMode& md = t->modes[mode];
+  const i64 nnz = t->nnz;
+  md.rows = t->shape[mode];
+  TENRSO_HIP_TRY(hipMalloc(&md.rowptr, (size_t)(md.rows + 1) * sizeof(int64_t)));
+  TENRSO_HIP_TRY(hipMalloc(&md.row, (size_t)std::max<i64>(1, nnz) * sizeof(int32_t)));
+  TENRSO_HIP_TRY(hipMalloc(&md.oth, (size_t)std::max<i64>(1, nnz * (t->nd - 1)) * sizeof(int32_t)));
+  TENRSO_HIP_TRY(hipMalloc(&md.vals, (size_t)std::max<i64>(1, nnz) * sizeof(float)));
+  if (nnz > 0) {
+    unsigned long long *kin, *kout;
+    int32_t *pin, *pout;
+    TENRSO_HIP_TRY(hipMalloc(&kin, (size_t)nnz * 8));
+    TENRSO_HIP_TRY(hipMalloc(&kout, (size_t)nnz * 8));
+    TENRSO_HIP_TRY(hipMalloc(&pin, (size_t)nnz * 4));
+    TENRSO_HIP_TRY(hipMalloc(&pout, (size_t)nnz * 4));
+    KeyArgs ka{};
+    ka.nd = t->nd;
+    ka.mode = mode;
+    ka.nnz = nnz;
+    for (int k = 0; k < t->nd; ++k) ka.shape[k] = t->shape[k];
+    ka.idx = t->idx;
+    hipLaunchKernelGGL(make_keys_kernel, dim3(cdiv(nnz, 256)), dim3(256), 0, ctx->stream, ka, kin, pin);
+    TENRSO_HIP_TRY(hipGetLastError());
+    int bits = 0;
+    {
+      unsigned long long maxkey = 1;
+      for (int k = 0; k < t->nd; ++k) maxkey *= (unsigned long long)t->shape[k];
+      while (bits < 64 && (maxkey >> bits) != 0) ++bits;
+    }
+    size_t tmp_bytes = 0;
+    TENRSO_HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, pin, pout, (size_t)nnz, 0, bits,
+                                             ctx->stream));
+    void* tmp = nullptr;
+    TENRSO_HIP_TRY(hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16)));
+    TENRSO_HIP_TRY(rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, pin, pout, (size_t)nnz, 0, bits,
+                                             ctx->stream));
+    GatherArgs ga{t->nd, mode, nnz, t->idx, t->vals, pout, md.row, md.oth, md.vals};
+    hipLaunchKernelGGL(gather_sorted_kernel, dim3(cdiv(nnz, 256)), dim3(256), 0, ctx->stream, ga);
+    TENRSO_HIP_TRY(hipGetLastError());
+    TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    TENRSO_HIP_TRY(hipFree(tmp));
+    TENRSO_HIP_TRY(hipFree(kin));
+    TENRSO_HIP_TRY(hipFree(kout));
+    TENRSO_HIP_TRY(hipFree(pin));
+    TENRSO_HIP_TRY(hipFree(pout));
+  }
+  hipLaunchKernelGGL(rowptr_kernel, dim3(cdiv(md.rows + 1, 256)), dim3(256), 0, ctx->stream, md.row, nnz, md.rows,
+                     md.rowptr);
+  TENRSO_HIP_TRY(hipGetLastError());
+  return TENRSO_OK;
+}
+
+int sp_create(tenrso_ctx* ctx, int nd, const i64* shape, i64 nnz, const int32_t* idx, const float* vals,
+              bool host, tenrso_sptensor** out) {
+  *out = nullptr;
+  if (nd < 1 || nd > MAXD_SP) {
+    set_error("sparse tensors need 1..%d modes", MAXD_SP);
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  // CooTensor::new validation (coo.rs:101-149)
+  for (int k = 0; k < nd; ++k)
+    if (shape[k] <= 0) {
+      set_error("Invalid shape: Shape cannot contain zeros");
+      return TENRSO_ERR_INVALID_ARGUMENT;
+    }
+  if (nnz < 0 || nnz > (i64)INT32_MAX) {
+    set_error("nnz %lld out of the supported range", (long long)nnz);
+    return TENRSO_ERR_UNSUPPORTED;
+  }
+  {
+    long double prod = 1.0L;
+    for (int k = 0; k < nd; ++k) prod *= (long double)shape[k];
+    if (prod >= 18446744073709551615.0L) {
+      set_error("tensor too large for 64-bit sort keys");
+      return TENRSO_ERR_UNSUPPORTED;
+    }
+    for (int k = 0; k < nd; ++k)
+      if (shape[k] > (i64)INT32_MAX) {
+        set_error("mode size exceeds the int32 index range");
+        return TENRSO_ERR_UNSUPPORTED;
+      }
+  }
+  auto* t = new tenrso_sptensor();
+  t->nd = nd;
+  t->nnz = nnz;
+  for (int k = 0; k < nd; ++k) t->shape[k] = shape[k];
+  int rc = TENRSO_OK;
+  auto fail = [&](int code) {
+    sp_destroy(t);
+    return code;
+  };
+  if (hipMalloc(&t->idx, (size_t)std::max<i64>(1, nnz * nd) * 4) != hipSuccess ||
+      hipMalloc(&t->vals, (size_t)std::max<i64>(1, nnz) * 4) != hipSuccess) {
+    set_error("device allocation failed for the sparse tensor");
+    return fail(TENRSO_ERR_DEVICE);
+  }
+  const hipMemcpyKind kind = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  if (nnz > 0) {
+    if (hipMemcpyAsync(t->idx, idx, (size_t)nnz * nd * 4, kind, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(t->vals, vals, (size_t)nnz * 4, kind, ctx->stream) != hipSuccess) {
+      set_error("copy of the COO arrays failed");
+      return fail(TENRSO_ERR_DEVICE);
+    }
+    // index bounds (CooTensor::new, coo.rs:127-139)
+    int* bad = nullptr;
+    if (hipMalloc(&bad, sizeof(int)) != hipSuccess) return fail(TENRSO_ERR_DEVICE);
+    (void)hipMemsetAsync(bad, 0, sizeof(int), ctx->stream);
+    KeyArgs ka{};
+    ka.nd = nd;
+    ka.nnz = nnz;
+    for (int k = 0; k < nd; ++k) ka.shape[k] = shape[k];
+    ka.idx = t->idx;
+    hipLaunchKernelGGL(bounds_kernel, dim3(cdiv(nnz, 256)), dim3(256), 0, ctx->stream, ka, bad);
+    int hbad = 0;
+    (void)hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(bad);
+    if (hbad) {
+      set_error("Index out of bounds");
+      return fail(TENRSO_ERR_INVALID_ARGUMENT);
+    }
+  }
+  for (int m = 0; m < nd; ++m) {
+    rc = sp_build_mode(ctx, t, m);
+    if (rc != TENRSO_OK) return fail(rc);
+  }
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    set_error("sparse build failed");
+    return fail(TENRSO_ERR_DEVICE);
+  }
+  *out = t;
+  return TENRSO_OK;
+}
+
+void sp_destroy(tenrso_sptensor* t) {
+  if (!t) return;
+  for (int m = 0; m < MAXD_SP; ++m) {
+    SpMode& md = t->modes[m];
+    if (md.rowptr) (void)hipFree(md.rowptr);
+    if (md.row) (void)hipFree(md.row);
+    if (md.oth) (void)hipFree(md.oth);
+    if (md.vals) (void)hipFree(md.vals);
+  }
+  if (t->idx) (void)hipFree(t->idx);
+  if (t->vals) (void)hipFree(t->vals);
+  delete t;
+}
+
+// ---------------------------------------------------------------- MTTKRP
+constexpr int SP_CH = 1024;  // nonzeros per worker chunk
+constexpr int SP_U = 4;      // nonzeros in flight per worker step
+
+struct SpArgs {
+  int nd_oth;
+  i64 nnz, rows, R;
+  int r0, rcount;
+  const int32_t* row;
+  const int32_t* oth;
+  const float* vals;
+  const float* F[MAXD_SP];  // factors of the other modes, ascending mode order
+  i64 nworkers;
+};
+
+template <int LPN>
+__global__ __launch_bounds__(256) void sp_mttkrp_kernel(SpArgs a, float* __restrict__ out,
+                                                        float* __restrict__ cval, int32_t* __restrict__ crow) {
+  const i64 gt = (i64)blockIdx.x * 256 + threadIdx.x;
+  const i64 w = gt / LPN;
+  const int lane = (int)(gt - w * LPN);
+  if (w >= a.nworkers) return;
+  const bool live = lane < a.rcount;
+  const int col = a.r0 + (live ? lane : 0);
+  const i64 e0 = w * SP_CH;
+  const i64 e1 = (e0 + SP_CH) < a.nnz ? e0 + SP_CH : a.nnz;
+  float* c0 = cval + (2 * w) * a.R;
+  float* c1 = cval + (2 * w + 1) * a.R;
+  const int first = a.row[e0];
+  int cur = first;
+  float acc = 0.f;
+  for (i64 e = e0; e < e1; e += SP_U) {
+    int rr[SP_U];
+    float v[SP_U];
+#pragma unroll
+    for (int u = 0; u < SP_U; ++u) {
+      const i64 ee = e + u;
+      const bool ok = ee < e1;
+      const i64 es = ok ? ee : e0;
+      rr[u] = ok ? a.row[es] : cur;
+      float p = ok ? a.vals[es] : 0.f;
+      for (int m = 0; m < a.nd_oth; ++m) {
+        const i64 ix = a.oth[(i64)m * a.nnz + es];
+        p *= a.F[m][ix * a.R + col];
+      }
+      v[u] = p;
+    }
+#pragma unroll
+    for (int u = 0; u < SP_U; ++u) {
+      if (rr[u] != cur) {
+        // flush row `cur`: the chunk's first row goes to carry 0, later rows are
+        // interior (the row changed again inside this chunk) and stored directly
+        if (cur == first) {
+          if (live) c0[col] = acc;
+        } else if (live) {
+          out[(i64)cur * a.R + col] = acc;
+        }
+        cur = rr[u];
+        acc = 0.f;
+      }
+      acc += v[u];
+    }
+  }
+  // The chunk's last row is a carry.  A single-row chunk emits (first, acc) and
+  // (first, 0), so the carry keys stay sorted with no gaps.
+  if (cur == first) {
+    if (live) {
+      c0[col] = acc;
+      c1[col] = 0.f;
+    }
+  } else if (live) {
+    c1[col] = acc;
+  }
+  if (lane == 0) {
+    crow[2 * w] = first;
+    crow[2 * w + 1] = cur;
+  }
+}
+
+// Segmented reduction of sorted (key, R-vector) slots, one LPN-lane worker per
+// tile of SP_TILE slots: keys strictly inside a tile are complete and stored to
+// `out`; the tile's first and last keys go to the next level's slots (2 per
+// tile).  With `final` set a single worker stores every key.  Fixed order, so
+// the result is deterministic.
+constexpr int SP_TILE = 32;
+
+template <int LPN>
+__global__ __launch_bounds__(256) void sp_segreduce_kernel(const float* __restrict__ ival, const int32_t* __restrict__ ikey,
+                                                           i64 S, i64 R, int r0, int rcount, int final_pass,
+                                                           float* __restrict__ out, float* __restrict__ oval,
+                                                           int32_t* __restrict__ okey) {
+  const i64 gt = (i64)blockIdx.x * 256 + threadIdx.x;
+  const i64 w = gt / LPN;
+  const int lane = (int)(gt - w * LPN);
+  const i64 tile = final_pass ? S : SP_TILE;
+  const i64 nw = final_pass ? 1 : (S + SP_TILE - 1) / SP_TILE;
+  if (w >= nw) return;
+  const bool live = lane < rcount;
+  const int col = r0 + (live ? lane : 0);
+  const i64 k0 = w * tile, k1 = (k0 + tile) < S ? k0 + tile : S;
+  const int first = ikey[k0];
+  int cur = first;
+  double acc = 0.0;
+  for (i64 k = k0; k < k1; ++k) {
+    const int key = ikey[k];
+    if (key != cur) {
+      if (cur == first && !final_pass) {
+        if (live) oval[(2 * w) * R + col] = (float)acc;
+      } else if (live) {
+        out[(i64)cur * R + col] = (float)acc;
+      }
+      cur = key;
+      acc = 0.0;
+    }
+    acc += (double)ival[k * R + col];
+  }
+  if (final_pass) {
+    if (live) out[(i64)cur * R + col] = (float)acc;
+    return;
+  }
+  if (cur == first) {
+    if (live) {
+      oval[(2 * w) * R + col] = (float)acc;
+      oval[(2 * w + 1) * R + col] = 0.f;
+    }
+  } else if (live) {
+    oval[(2 * w + 1) * R + col] = (float)acc;
+  }
+  if (lane == 0) {
+    okey[2 * w] = first;
+    okey[2 * w + 1] = cur;
+  }
+}
+
+__global__ __launch_bounds__(256) void zero_f32_kernel(float* __restrict__ p, i64 n) {
+  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+template <int LPN>
+static int sp_mttkrp_launch(tenrso_ctx* ctx, SpArgs& a, float* out, float* cv0, int32_t* ck0, float* cv1,
+                            int32_t* ck1) {
+  const i64 R = a.R;
+  hipLaunchKernelGGL(sp_mttkrp_kernel<LPN>, dim3(cdiv(a.nworkers * LPN, 256)), dim3(256), 0, ctx->stream, a, out,
+                     cv0, ck0);
+  i64 S = 2 * a.nworkers;
+  float* iv = cv0;
+  int32_t* ik = ck0;
+  float* ov = cv1;
+  int32_t* ok = ck1;
+  while (S > SP_TILE) {
+    const i64 nw = (S + SP_TILE - 1) / SP_TILE;
+    hipLaunchKernelGGL(sp_segreduce_kernel<LPN>, dim3(cdiv(nw * LPN, 256)), dim3(256), 0, ctx->stream,
+                       (const float*)iv, (const int32_t*)ik, S, R, a.r0, a.rcount, 0, out, ov, ok);
+    S = 2 * nw;
+    std::swap(iv, ov);
+    std::swap(ik, ok);
+  }
+  hipLaunchKernelGGL(sp_segreduce_kernel<LPN>, dim3(cdiv(LPN, 256)), dim3(256), 0, ctx->stream, (const float*)iv,
+                     (const int32_t*)ik, S, R, a.r0, a.rcount, 1, out, ov, ok);
+  TENRSO_HIP_TRY(hipGetLastError());
+  return TENRSO_OK;
+}
+
+int sp_mttkrp(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* F, i64 R, int mode, float* out) {
+  const SpMode& md = t->modes[mode];
+  const i64 I = md.rows;
+  if (I * R > 0)
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(cdiv(I * R, 256)), dim3(256), 0, ctx->stream, out, I * R);
+  TENRSO_HIP_TRY(hipGetLastError());
+  if (t->nnz == 0 || R == 0) return TENRSO_OK;
+  SpArgs a{};
+  a.nd_oth = t->nd - 1;
+  a.nnz = t->nnz;
+  a.rows = I;
+  a.R = R;
+  a.row = md.row;
+  a.oth = md.oth;
+  a.vals = md.vals;
+  int o = 0;
+  for (int k = 0; k < t->nd; ++k)
+    if (k != mode) a.F[o++] = F[k];
+  a.nworkers = (t->nnz + SP_CH - 1) / SP_CH;
+  const i64 nslots = 2 * a.nworkers;
+  void *pv, *pk;
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_TMP, (size_t)2 * nslots * R * sizeof(float), &pv));
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_TMP2, (size_t)2 * nslots * sizeof(int32_t), &pk));
+  float* cv0 = (float*)pv;
+  float* cv1 = cv0 + nslots * R;
+  int32_t* ck0 = (int32_t*)pk;
+  int32_t* ck1 = ck0 + nslots;
+  const int LPN = R <= 16 ? 16 : (R <= 32 ? 32 : 64);
+  const int rchunks = (int)((R + LPN - 1) / LPN);
+  for (int rc = 0; rc < rchunks; ++rc) {
+    a.r0 = rc * LPN;
+    a.rcount = (int)std::min<i64>(LPN, R - a.r0);
+    if (LPN == 16) TENRSO_TRY(sp_mttkrp_launch<16>(ctx, a, out, cv0, ck0, cv1, ck1));
+    else if (LPN == 32) TENRSO_TRY(sp_mttkrp_launch<32>(ctx, a, out, cv0, ck0, cv1, ck1));
+    else TENRSO_TRY(sp_mttkrp_launch<64>(ctx, a, out, cv0, ck0, cv1, ck1));
+  }
+  return TENRSO_OK;
+}
+
+// ---------------------------------------------------------------- CSF export
+// Standard CSF (SPLATT semantics) of mode order [mode, others ascending] from the
+// sorted nonzeros: level l fibers are the runs of equal (i_0..i_l) prefixes;
+// fptr[l][f] .. fptr[l][f+1] index the children at level l+1 (the leaf level
+// indexes nonzeros).  Built on the host from the device-sorted arrays (a format
+// export for callers and tests, not on the MTTKRP path).
+int sp_export_csf(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, std::vector<std::vector<int64_t>>& fptr,
+                  std::vector<std::vector<int32_t>>& fids, std::vector<float>& vals) {
+  const SpMode& md = t->modes[mode];
+  const i64 nnz = t->nnz;
+  const int nd = t->nd;
+  std::vector<std::vector<int32_t>> lvl(nd, std::vector<int32_t>((size_t)nnz));
+  vals.assign((size_t)nnz, 0.f);
+  if (nnz > 0) {
+    TENRSO_HIP_TRY(hipMemcpyAsync(lvl[0].data(), md.row, (size_t)nnz * 4, hipMemcpyDeviceToHost, ctx->stream));
+    for (int l = 1; l < nd; ++l)
+      TENRSO_HIP_TRY(hipMemcpyAsync(lvl[l].data(), md.oth + (i64)(l - 1) * nnz, (size_t)nnz * 4,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    TENRSO_HIP_TRY(hipMemcpyAsync(vals.data(), md.vals, (size_t)nnz * 4, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  fptr.assign(nd, {});
+  fids.assign(nd, {});
+  // level starts: positions where the prefix (levels 0..l) changes
+  std::vector<std::vector<int64_t>> starts(nd);
+  for (int l = 0; l < nd; ++l) {
+    for (i64 e = 0; e < nnz; ++e) {
+      bool change = (e == 0);
+      for (int q = 0; q <= l && !change; ++q) change = lvl[q][(size_t)e] != lvl[q][(size_t)e - 1];
+      if (change || l == nd - 1) starts[l].push_back(e);
+    }
+    for (int64_t s : starts[l]) fids[l].push_back(lvl[l][(size_t)s]);
+  }
+  for (int l = 0; l < nd; ++l) {
+    if (l == nd - 1) {
+      fptr[l].push_back(0);  // leaf level: one entry per nonzero, no children
+      continue;
+    }
+    // children of fiber f at level l are the level-(l+1) fibers starting inside it
+    const auto& cs = starts[l + 1];
+    size_t ci = 0;
+    for (size_t f = 0; f < starts[l].size(); ++f) {
+      fptr[l].push_back((int64_t)ci);
+      const int64_t end = (f + 1 < starts[l].size()) ? starts[l][f + 1] : nnz;
+      while (ci < cs.size() && cs[ci] < end) ++ci;
+    }
+    fptr[l].push_back((int64_t)ci);
+  }
+  return TENRSO_OK;
+}
+
+}  // namespace tenrso

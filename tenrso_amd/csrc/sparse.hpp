@@ -1,0 +1,40 @@
+// sparse.hpp -- device sparse tensor (per-mode sorted COO + CSF export) and sparse MTTKRP.
+#pragma once
+#include <vector>
+
+#include "common.hpp"
+
+namespace tenrso {
+
+constexpr int MAXD_SP = 8;
+
+struct SpMode {
+  i64 rows = 0;                // I_n
+  int64_t* rowptr = nullptr;   // I_n + 1 nonzero offsets per output row
+  int32_t* row = nullptr;      // nnz: output-mode index of each sorted nonzero
+  int32_t* oth = nullptr;      // (nd-1) x nnz: other modes' indices, ascending mode order
+  float* vals = nullptr;       // nnz
+};
+
+}  // namespace tenrso
+
+struct tenrso_sptensor {
+  int nd = 0;
+  int64_t nnz = 0;
+  int64_t shape[tenrso::MAXD_SP] = {};
+  int32_t* idx = nullptr;  // original COO, SoA nd x nnz
+  float* vals = nullptr;
+  tenrso::SpMode modes[tenrso::MAXD_SP];
+};
+
+namespace tenrso {
+
+int sp_create(tenrso_ctx* ctx, int nd, const i64* shape, i64 nnz, const int32_t* idx, const float* vals, bool host,
+              tenrso_sptensor** out);
+void sp_destroy(tenrso_sptensor* t);
+// M = sparse MTTKRP (fp32, shape[mode] x R), deterministic
+int sp_mttkrp(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* F, i64 R, int mode, float* out);
+int sp_export_csf(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, std::vector<std::vector<int64_t>>& fptr,
+                  std::vector<std::vector<int32_t>>& fids, std::vector<float>& vals);
+
+}  // namespace tenrso
