@@ -190,6 +190,34 @@ int tenrso_cp_als_f32_host(tenrso_ctx* ctx, const float* X, int ndim, const int6
                            double* fit_history);
 
 /* ------------------------------------------------------------------------
+ * Reverse mode.  Device pointers throughout; rank <= 64 for the FP64 Gram paths.
+ * ------------------------------------------------------------------------ */
+/* VJP of mttkrp (no reference counterpart; tenrso-ad registers custom rules through
+ * VjpOp, tenrso-ad/src/vjp.rs:30-44).  For the cotangent Mbar (shape[mode] x rank):
+ *   dfactors[m] = mttkrp(X, factors with factors[mode] := Mbar, m) for m != mode
+ *   dX          = cp_reconstruct(factors with factors[mode] := Mbar)
+ * dfactors[m] (or the whole array) and dX may be NULL to skip them; dfactors[mode]
+ * is not written (the mode's own factor is not an input of the MTTKRP). */
+int tenrso_mttkrp_vjp_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                          const float* const* factors, int64_t rank, int mode, const float* cotangent,
+                          float* const* dfactors, float* dX);
+/* CpReconstructionGrad::compute_factor_gradients (tenrso-ad/src/grad.rs:81-120):
+ * grads[n] = mttkrp(grad_output, factors, n) * diag(weights) (weights may be NULL). */
+int tenrso_cp_factor_grads_f32(tenrso_ctx* ctx, const float* grad_output, int ndim, const int64_t* shape,
+                               const float* const* factors, int64_t rank, const float* weights,
+                               float* const* grads);
+/* Loss 0.5 ||X - [[A_1..A_N]]||^2 and grads[n] = A_n Gamma_n - mttkrp(X, A, n),
+ * Gamma_n = hadamard_{k != n} A_k^T A_k (Gram, Gamma, product and loss in FP64).
+ * grads may be NULL (loss only). */
+int tenrso_cp_loss_grad_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                            const float* const* factors, int64_t rank, float* const* grads, double* loss);
+/* Gradient-descent CP fit: `steps` simultaneous updates A_n -= lr * grad_n (lr <= 0:
+ * lr = 1 / max_n ||Gamma_n||_F at each step).  loss_history[s] (host, may be NULL)
+ * is the loss at the point step s starts from. */
+int tenrso_cp_gd_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, float* const* factors,
+                     int64_t rank, int steps, double lr, double* loss_history);
+
+/* ------------------------------------------------------------------------
  * Sparse tensors.  A COO tensor (tenrso-sparse/src/coo.rs:73-348) is uploaded
  * once and sorted on the device, per output mode, in the lexicographic order of
  * CsfTensor::from_coo with mode_order [mode, others ascending] (csf.rs:137-264).

@@ -66,6 +66,10 @@ SIGNATURES = {
     "tenrso_cp_reconstruct_f32": ([_vp, _i, _vp, _vp, _i64, _vp, _vp], _i),
     "tenrso_cp_als_f32": ([_vp, _vp, _i, _vp, _vp, C.POINTER(CpOptions), C.POINTER(CpResult), _vp], _i),
     "tenrso_cp_als_f32_host": ([_vp, _vp, _i, _vp, _vp, C.POINTER(CpOptions), C.POINTER(CpResult), _vp], _i),
+    "tenrso_mttkrp_vjp_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _i, _vp, _vp, _vp], _i),
+    "tenrso_cp_factor_grads_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _vp, _vp], _i),
+    "tenrso_cp_loss_grad_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _vp, C.POINTER(_d)], _i),
+    "tenrso_cp_gd_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _i, _d, _vp], _i),
     "tenrso_sp_create": ([_vp, _i, _vp, _i64, _vp, _vp, _i, C.POINTER(_vp)], _i),
     "tenrso_sp_destroy": ([_vp], _i),
     "tenrso_sp_mode_arrays": ([_vp, _vp, _i, _vp, _vp, _vp, _vp], _i),

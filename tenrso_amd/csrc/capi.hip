@@ -11,6 +11,7 @@
 #include "mttkrp.hpp"
 #include "cp_linalg.hpp"
 #include "sparse.hpp"
+#include "ad.hpp"
 
 using namespace tenrso;
 
@@ -438,6 +439,55 @@ int tenrso_sp_cp_als_f32(tenrso_ctx* ctx, const tenrso_sptensor* t, float* const
     return TENRSO_OK;
   };
   return cp_als_run(ctx, pb, factors, opts, result, fit_history);
+}
+
+// ---------------------------------------------------------------- reverse mode
+int tenrso_mttkrp_vjp_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                          const float* const* factors, int64_t rank, int mode, const float* cotangent,
+                          float* const* dfactors, float* dX) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  TENRSO_TRY(check_mttkrp_basic(ndim, shape, rank, mode));
+  if (rank == 0) return TENRSO_OK;
+  TENRSO_TRY(set_device(ctx));
+  return mttkrp_vjp(ctx, X, ndim, shape, factors, rank, mode, cotangent, dfactors, dX);
+}
+
+int tenrso_cp_factor_grads_f32(tenrso_ctx* ctx, const float* grad_output, int ndim, const int64_t* shape,
+                               const float* const* factors, int64_t rank, const float* weights,
+                               float* const* grads) {
+  clear_error();
+  if (bad_ctx(ctx) || !grads) return TENRSO_ERR_INVALID_ARGUMENT;
+  TENRSO_TRY(check_mttkrp_basic(ndim, shape, rank, 0));
+  if (rank == 0) return TENRSO_OK;
+  TENRSO_TRY(set_device(ctx));
+  return cp_factor_grads(ctx, grad_output, ndim, shape, factors, rank, weights, grads);
+}
+
+int tenrso_cp_loss_grad_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                            const float* const* factors, int64_t rank, float* const* grads, double* loss) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  TENRSO_TRY(check_mttkrp_basic(ndim, shape, rank, 0));
+  if (rank <= 0) {
+    set_error("Invalid rank: %lld", (long long)rank);
+    return TENRSO_ERR_INVALID_RANK;
+  }
+  TENRSO_TRY(set_device(ctx));
+  return cp_loss_grad(ctx, X, ndim, shape, factors, rank, grads, loss);
+}
+
+int tenrso_cp_gd_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, float* const* factors,
+                     int64_t rank, int steps, double lr, double* loss_history) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  TENRSO_TRY(check_mttkrp_basic(ndim, shape, rank, 0));
+  if (rank <= 0 || steps < 0) {
+    set_error("rank must be positive and steps non-negative");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  TENRSO_TRY(set_device(ctx));
+  return cp_gd(ctx, X, ndim, shape, factors, rank, steps, lr, loss_history);
 }
 
 }  // extern "C"
