@@ -78,6 +78,40 @@ def cpu_baseline(X, init, gpu_fits, max_iters):
     }
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mttkrp_256cube_r64_traffic.md")
+
+
+def pmc_probe(Xd, Fd, shape, R):
+    """Kernels for the HBM-traffic counter runs: a plain 256 MiB copy (known bytes,
+    calibrates FETCH_SIZE / WRITE_SIZE), then 3 launches of each MTTKRP kernel."""
+    import torch
+    from tenrso_amd import _lib, kernels
+    src = torch.ones(64 << 20, dtype=torch.float32, device=Xd.device)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    ctx = _lib.torch_context(Xd)
+    Xb = kernels.to_bf16(Xd)
+    for mode in range(len(shape)):
+        profile_kernel(ctx, _lib.PREC_FP32, Xd, shape, Fd, R, mode, 3)
+        profile_kernel(ctx, _lib.PREC_BF16, Xb, shape, Fd, R, mode, 3)
+    torch.cuda.synchronize()
+    print("pmc probe done", flush=True)
+
+
+def read_traffic():
+    """HBM bytes per launch measured by the committed rocprofv3 --pmc summary (a line
+    'TRAFFIC_JSON {...}'), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("TRAFFIC_JSON "):
+                    return json.loads(line[len("TRAFFIC_JSON "):])
+    except OSError:
+        return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,10 +119,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--reps", type=int, default=20, help="launches per MTTKRP kernel timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="no timing: launch a 256 MiB device copy (counter calibration) and each MTTKRP kernel a few "
+                         "times, for rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is described in "
-              "DESIGN.md and not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding across GPUs is not part of "
+              "this library", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -102,6 +139,9 @@ def main():
     Xd = torch.from_numpy(X).to(dev)
     Fd = [torch.from_numpy(f).to(dev) for f in init]
     K, W = args.steps, args.warmup
+    if args.pmc_probe:
+        pmc_probe(Xd, Fd, shape, R)
+        return
 
     def timed_als(precision):
         decomp.cp_als_with_init(Xd, Fd, max(W, 1), 0.0, precision=precision)  # warmup (W iterations)
@@ -135,15 +175,16 @@ def main():
     # dominant kernel: the fp32 MFMA instance of modes 0 and 1 (q-contiguous tiles)
     t_dom = (modes["0"]["fp32_us"] + modes["1"]["fp32_us"]) / 2.0
     ach = mttkrp_flops(shape, R) / (t_dom * 1e-6) / 1e12
+    traffic = read_traffic() or {}
     roofline = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
+                "frac": ach / PEAK_FP32_TFLOPS, "traffic": traffic.get("fp32_bytes"),
                 "kernel": "mttkrp_f32_mfma<4,4,2,false,128> (modes 0,1)", "avg_us": t_dom,
                 "algorithmic_flops": mttkrp_flops(shape, R)}
     t_b16 = (modes["0"]["bf16_us"] + modes["1"]["bf16_us"]) / 2.0
     b_alg = mttkrp_bytes(shape, R, 0, 2)
     roof_bf16 = {"bound": "hbm", "achieved": b_alg / (t_b16 * 1e-6) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                  "frac": b_alg / (t_b16 * 1e-6) / 1e9 / PEAK_HBM_GBS, "kernel": "mttkrp_bf16_mfma<4,4,2,2,128> (modes 0,1)",
-                 "avg_us": t_b16, "algorithmic_bytes": b_alg}
+                 "avg_us": t_b16, "algorithmic_bytes": b_alg, "traffic": traffic.get("bf16_bytes")}
 
     gpu_fits = {"fp32": cp32.fit, "bf16": cp16.fit}
     cpu = None

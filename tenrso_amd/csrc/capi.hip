@@ -11,6 +11,7 @@
 #include "mttkrp.hpp"
 #include "cp_linalg.hpp"
 #include "sparse.hpp"
+#include "als_job.hpp"
 #include "ad.hpp"
 
 using namespace tenrso;
@@ -428,7 +429,8 @@ int tenrso_sp_cp_als_f32(tenrso_ctx* ctx, const tenrso_sptensor* t, float* const
   pb.nd = t->nd;
   pb.shape = t->shape;
   pb.norm_min = [&](double* tmp, double* scal) { return tensor_norm_min(ctx, t->vals, t->nnz, tmp, scal); };
-  pb.mttkrp = [&](int mode, const float* const* F, MttkrpResult* mr) -> int {
+  pb.mttkrp = [&](int mode, const float* const* F, const AlsJob* job, MttkrpResult* mr) -> int {
+    if (job) TENRSO_TRY(als_job_launch(ctx, *job));
     i64 maxI = 0;
     for (int k = 0; k < t->nd; ++k) maxI = std::max<i64>(maxI, t->shape[k]);
     void* pm = nullptr;

@@ -56,13 +56,11 @@ struct tenrso_ctx {
   int num_cus = 256;
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipStream_t aux = nullptr;          // side stream (default priority) for work overlapped with MTTKRP
-  hipEvent_t ev_g = nullptr, ev_w = nullptr;  // ordering events between the two streams
+  hipEvent_t ev_fit[2] = {nullptr, nullptr};  // "fit of sweep k is in pinned[k % 2]" markers
   double last_kernel_ms = 0.0;
   int profile_reps = 0;               // > 0: mttkrp_dense times its main kernel
   int wg_per_cu_f32 = 2;              // MTTKRP grid sizing (TENRSO_WG_PER_CU_F32 / _BF16)
   int wg_per_cu_bf16 = 2;
-  bool serial_solve = false;          // TENRSO_SERIAL_SOLVE=1: no side stream (profiling aid)
   std::vector<tenrso::DevBuf> bufs;  // indexed by slot id
   void* pinned = nullptr;            // small pinned host buffer for scalar readback
 };
@@ -85,6 +83,7 @@ enum BufSlot : int {
   SLOT_TMP2,
   SLOT_IP,
   SLOT_SPM,
+  SLOT_FSET,
   SLOT_COUNT
 };
 

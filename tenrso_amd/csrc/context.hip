@@ -93,9 +93,8 @@ int tenrso_ctx_create(int device, int borrow_stream, void* stream, tenrso_ctx** 
     c->own_stream = true;
   }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_g, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_w, hipEventDisableTiming) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fit[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fit[1], hipEventDisableTiming) != hipSuccess ||
       hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
     set_error("context setup failed");
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -105,7 +104,6 @@ int tenrso_ctx_create(int device, int borrow_stream, void* stream, tenrso_ctx** 
   c->bufs.resize(SLOT_COUNT);
   if (const char* e = std::getenv("TENRSO_WG_PER_CU_F32")) c->wg_per_cu_f32 = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("TENRSO_WG_PER_CU_BF16")) c->wg_per_cu_bf16 = std::max(1, std::atoi(e));
-  if (const char* e = std::getenv("TENRSO_SERIAL_SOLVE")) c->serial_solve = std::atoi(e) != 0;
   *out = c;
   return TENRSO_OK;
 }
@@ -118,12 +116,8 @@ int tenrso_ctx_destroy(tenrso_ctx* c) {
     if (b.ptr) (void)hipFree(b.ptr);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->ev_g) (void)hipEventDestroy(c->ev_g);
-  if (c->ev_w) (void)hipEventDestroy(c->ev_w);
-  if (c->aux) {
-    (void)hipStreamSynchronize(c->aux);
-    (void)hipStreamDestroy(c->aux);
-  }
+  for (hipEvent_t e : c->ev_fit)
+    if (e) (void)hipEventDestroy(e);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -3,20 +3,29 @@
 // Restates the control flow of tenrso_decomp::cp_als (cp/core.rs:59-182) and
 // cp_als_constrained (cp/core.rs:225-408): the same validation order, the same
 // Gauss-Seidel mode sweep, the same fit / oscillation / tolerance / time-limit
-// rules.  The arithmetic runs on the GPU:
-//   per mode : MTTKRP (fp32 or bf16 MFMA, split partials summed in FP64)
-//              -> Gamma^{-1} (FP64, one workgroup)
-//              -> rows A_n = epi(M Gamma^{-1}) (FP64, written as fp32)
-//              -> Gram A_n^T A_n (FP64) kept for the next modes and the fit
-//   per iter : fit from the Grams and <X, Xhat> = sum M_{N-1} . A_{N-1}
-//              (the last mode's MTTKRP, which equals the reference's separate
-//              mode-0 MTTKRP of cp/helpers.rs:558-579 in exact arithmetic).
-// The only host<->device traffic per iteration is the scalar block read back
-// for the convergence test.
+// rules.  The arithmetic runs on the GPU, per mode n of a sweep:
+//   one launch   MTTKRP of mode n (fp32 or bf16 MFMA, split partials) whose
+//                workgroup 0 runs the FP64 job: the previous sweep's fit (at
+//                n = 0), Gamma_n and its inverse -- so the latency-bound linear
+//                algebra shares the GPU with the bandwidth-bound MTTKRP instead
+//                of waiting behind it;
+//   one launch   rows A_n = epi(M Gamma_n^{-1}) (FP64, written as fp32) and the
+//                <M, A_n> partials;
+//   one launch   the Gram A_n^T A_n (FP64) for the modes that follow.
+// Fit: from the Grams and <X, Xhat> = sum M_{N-1} . A_{N-1} (the last mode's
+// MTTKRP, equal in exact arithmetic to the reference's separate mode-0 MTTKRP of
+// cp/helpers.rs:558-579, which fit_from_mode0 selects).
+//
+// Sweep k reads factor set k % 2 and writes set (k + 1) % 2 (set 0 is the
+// caller's factors, set 1 context scratch).  So the host can queue sweep k + 1
+// before it has read the fit of sweep k: if sweep k turns out to be the last,
+// sweep k + 1 has not touched its result.  The only host<->device traffic per
+// sweep is the scalar block copied back for the convergence test.
 #include <chrono>
 #include <cmath>
 #include <cstring>
 
+#include "als_job.hpp"
 #include "cp_als.hpp"
 #include "cp_linalg.hpp"
 #include "mttkrp.hpp"
@@ -54,8 +63,8 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
     set_error("tensors with more than %d modes are not supported", MAXD);
     return TENRSO_ERR_UNSUPPORTED;
   }
-  if (R > SOLVE_RMAX) {
-    set_error("rank %lld exceeds the on-device normal-equation solve bound %d", (long long)R, SOLVE_RMAX);
+  if (R > JOB_NE) {
+    set_error("rank %lld exceeds the on-device normal-equation solve bound %d", (long long)R, JOB_NE);
     return TENRSO_ERR_UNSUPPORTED;
   }
   if (o->orthogonal) {
@@ -63,31 +72,48 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
     return TENRSO_ERR_UNSUPPORTED;
   }
   TENRSO_TRY(set_device(ctx));
-  i64 maxI = 0;
-  for (int k = 0; k < nd; ++k) maxI = std::max(maxI, shape[k]);
+  i64 maxI = 0, ftot = 0;
+  for (int k = 0; k < nd; ++k) {
+    maxI = std::max(maxI, shape[k]);
+    ftot += shape[k] * R;
+  }
 
   // ---- device workspace
-  void *p_scal, *p_G, *p_Winv, *p_M, *p_tmp;
+  void *p_scal, *p_G, *p_Winv, *p_tmp, *p_set;
   TENRSO_TRY(ctx_buffer(ctx, SLOT_SCALARS, SC_COUNT * sizeof(double), &p_scal));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_GRAMS, (size_t)(nd + 1) * R * R * sizeof(double), &p_G));
-  TENRSO_TRY(ctx_buffer(ctx, SLOT_SOLVE, (size_t)SOLVE_RMAX * SOLVE_RMAX * sizeof(double), &p_Winv));
-  TENRSO_TRY(ctx_buffer(ctx, SLOT_M64, (size_t)maxI * R * sizeof(double), &p_M));
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_SOLVE, (size_t)JOB_NE * JOB_NE * sizeof(double), &p_Winv));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_IP, (size_t)std::max<i64>(4096, maxI + 16) * sizeof(double), &p_tmp));
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_FSET, (size_t)ftot * sizeof(float), &p_set));
   double* scal = (double*)p_scal;
   double* G = (double*)p_G;
   double* Winv = (double*)p_Winv;
-  double* M = (double*)p_M;
   double* ip_part = (double*)p_tmp;
+  double* M = nullptr;
+  if (o->fit_from_mode0) {
+    void* p_M;
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_M64, (size_t)maxI * R * sizeof(double), &p_M));
+    M = (double*)p_M;
+  }
+  float* set[2][MAXD];
+  {
+    i64 off = 0;
+    for (int k = 0; k < nd; ++k) {
+      set[0][k] = F[k];
+      set[1][k] = (float*)p_set + off;
+      off += shape[k] * R;
+    }
+  }
 
   TENRSO_TRY(fill_f64(ctx, scal, SC_COUNT, 0.0));
   TENRSO_TRY(pb.norm_min(ip_part, scal));
 
   const bool nonneg = o->nonnegative != 0;
+  double* pinned = (double*)ctx->pinned;  // [2][SC_COUNT]
   if (nonneg && o->validate_nonneg_input) {
-    double* hs = (double*)ctx->pinned;
-    TENRSO_HIP_TRY(hipMemcpyAsync(hs, scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TENRSO_HIP_TRY(hipMemcpyAsync(pinned, scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (hs[SC_MIN_X] < 0.0) {
+    if (pinned[SC_MIN_X] < 0.0) {
       set_error("Non-negative constraint violated: input tensor contains negative values");
       return TENRSO_ERR_NONNEGATIVE;
     }
@@ -98,79 +124,139 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
 
   for (int k = 0; k < nd; ++k) TENRSO_TRY(gram_f64(ctx, F[k], shape[k], R, G + (i64)k * R * R));
 
-  GramReg reg;
-  reg.l2 = o->l2;
-  reg.tikhonov_order = o->tikhonov_order;
-  reg.tikhonov_lambda = o->tikhonov_lambda;
   RowEpilogue epi;
   epi.l1 = o->l1;
   epi.nonneg = nonneg ? 1 : 0;
+  int nblk = 0;  // <M, A> partials written by the last row solve
 
-  double prev_fit = 0.0, fit = 0.0, final_change = 0.0;
-  int iters = 0, osc = 0, reason = TENRSO_REASON_MAX_ITERATIONS, worst = 0;
-  const float* const* Fc = (const float* const*)F;
-  const auto t0 = std::chrono::steady_clock::now();
-  for (int iter = 0; iter < o->max_iters; ++iter) {
-    if (o->time_limit_s >= 0.0) {
-      const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      if (el > o->time_limit_s) {
-        reason = TENRSO_REASON_TIME_LIMIT;
-        break;
-      }
-    }
-    iters = iter + 1;
-    int nblk = 0;
-    for (int mode = 0; mode < nd; ++mode) {
-      const i64 I = shape[mode];
-      // Side stream: the Gram of the factor updated just before (mode - 1) and
-      // Gamma_n^{-1}; both only need factors that are final for this mode, so
-      // they run while the MTTKRP of mode n runs on the main stream.
-      TENRSO_HIP_TRY(hipEventRecord(ctx->ev_g, ctx->stream));
-      hipStream_t side = ctx->serial_solve ? ctx->stream : ctx->aux;
-      if (!ctx->serial_solve) TENRSO_HIP_TRY(hipStreamWaitEvent(ctx->aux, ctx->ev_g, 0));
-      if (mode > 0) TENRSO_TRY(gram_f64(ctx, F[mode - 1], shape[mode - 1], R, G + (i64)(mode - 1) * R * R, side));
-      TENRSO_TRY(normal_eq_inverse(ctx, G, nd, mode, R, reg, Winv, scal, side));
-      TENRSO_HIP_TRY(hipEventRecord(ctx->ev_w, side));
+  auto base_job = [&]() {
+    AlsJob j{};
+    j.G = G;
+    j.nd = nd;
+    j.R = (int)R;
+    j.ip_part = ip_part;
+    j.nblk = nblk;
+    j.skip = -1;
+    j.l2 = o->l2;
+    j.tik_order = o->tikhonov_order;
+    j.tik_lambda = o->tikhonov_lambda;
+    j.Winv = Winv;
+    j.scal = scal;
+    j.status_slot = SC_STATUS;
+    return j;
+  };
+  // copy the scalar block to pinned[k % 2] and mark it
+  auto publish = [&](int k) -> int {
+    TENRSO_HIP_TRY(hipMemcpyAsync(pinned + (k & 1) * SC_COUNT, scal, SC_COUNT * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    TENRSO_HIP_TRY(hipEventRecord(ctx->ev_fit[k & 1], ctx->stream));
+    return TENRSO_OK;
+  };
+  // queue sweep k; with fit_prev its first job also evaluates the fit of sweep k - 1
+  auto enqueue_sweep = [&](int k, bool fit_prev) -> int {
+    float* const* olds = set[k & 1];
+    float* const* news = set[(k + 1) & 1];
+    for (int n = 0; n < nd; ++n) {
+      const float* Fc[MAXD];
+      for (int m = 0; m < nd; ++m) Fc[m] = m < n ? news[m] : olds[m];
+      AlsJob job = base_job();
+      job.do_fit = (n == 0 && fit_prev) ? 1 : 0;
+      job.skip = n;
+      job.status_slot = (k & 1) ? SC_STATUS_ODD : SC_STATUS;
+      job.status_reset = n == 0 ? 1 : 0;
       MttkrpResult mr;
-      TENRSO_TRY(pb.mttkrp(mode, Fc, &mr));
-      if (!ctx->serial_solve) TENRSO_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_w, 0));
-      TENRSO_TRY(solve_rows(ctx, mr.partial, mr.splits, nullptr, I, R, Winv, epi, F[mode], ip_part, &nblk));
+      TENRSO_TRY(pb.mttkrp(n, Fc, &job, &mr));
+      if (n == 0 && fit_prev) TENRSO_TRY(publish(k - 1));
+      TENRSO_TRY(solve_rows(ctx, mr.partial, mr.splits, nullptr, shape[n], R, Winv, epi, news[n], ip_part, &nblk));
+      TENRSO_TRY(gram_f64(ctx, news[n], shape[n], R, G + (i64)n * R * R));
     }
-    // Gram of the last factor (needed by the fit and by the next iteration)
-    TENRSO_TRY(gram_f64(ctx, F[nd - 1], shape[nd - 1], R, G + (i64)(nd - 1) * R * R));
+    return TENRSO_OK;
+  };
+  // fit of sweep k - 1 on its own (no further sweep queued behind it)
+  auto enqueue_fit = [&](int k) -> int {
+    float* const* cur = set[k & 1];
     if (o->fit_from_mode0) {
       // cp/helpers.rs:558-579: <X, Xhat> from a mode-0 MTTKRP with every updated factor
       MttkrpResult mr;
-      TENRSO_TRY(pb.mttkrp(0, Fc, &mr));
+      TENRSO_TRY(pb.mttkrp(0, (const float* const*)cur, nullptr, &mr));
       TENRSO_TRY(reduce_partials(ctx, mr.partial, mr.splits, shape[0] * R, nullptr, M));
-      TENRSO_TRY(inner_rows(ctx, M, F[0], shape[0], R, ip_part, &nblk));
+      TENRSO_TRY(inner_rows(ctx, M, cur[0], shape[0], R, ip_part, &nblk));
     }
-    TENRSO_TRY(fit_from_grams(ctx, G, nd, R, ip_part, nblk, scal));
-    double* hs = (double*)ctx->pinned;
-    TENRSO_HIP_TRY(hipMemcpyAsync(hs, scal, SC_COUNT * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    worst = std::max(worst, (int)hs[SC_STATUS]);
-    if (worst >= 2) {
-      set_error("Linear algebra error: the Gram-Hadamard matrix is not positive definite after the ridge retry");
-      return TENRSO_ERR_LINALG;
+    AlsJob job = base_job();
+    job.do_fit = 1;
+    TENRSO_TRY(als_job_launch(ctx, job));
+    return publish(k - 1);
+  };
+
+  // speculation (queue sweep k before reading the fit of k - 1) keeps the GPU
+  // busy through the host's convergence test; with a time limit or the
+  // reference fit form each sweep is queued only after that test.
+  const bool spec = !o->fit_from_mode0 && !(o->time_limit_s >= 0.0);
+  double prev_fit = 0.0, fit = 0.0, final_change = 0.0;
+  int iters = 0, osc = 0, reason = TENRSO_REASON_MAX_ITERATIONS, worst = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto time_up = [&]() {
+    if (!(o->time_limit_s >= 0.0)) return false;
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > o->time_limit_s;
+  };
+  int status = TENRSO_OK;
+  if (o->max_iters > 0) {
+    if (time_up()) {
+      reason = TENRSO_REASON_TIME_LIMIT;
+    } else {
+      TENRSO_TRY(enqueue_sweep(0, false));
+      for (int k = 1;; ++k) {
+        // sweep k - 1 is queued
+        const bool more = spec && k < o->max_iters;
+        if (more) TENRSO_TRY(enqueue_sweep(k, true));
+        else TENRSO_TRY(enqueue_fit(k));
+        TENRSO_HIP_TRY(hipEventSynchronize(ctx->ev_fit[(k - 1) & 1]));
+        const double* hs = pinned + ((k - 1) & 1) * SC_COUNT;
+        iters = k;
+        const int it = k - 1;
+        worst = std::max(worst, (int)hs[(it & 1) ? SC_STATUS_ODD : SC_STATUS]);
+        if (worst >= 2) {
+          set_error("Linear algebra error: the Gram-Hadamard matrix is not positive definite after the ridge retry");
+          status = TENRSO_ERR_LINALG;
+          break;
+        }
+        fit = hs[SC_FIT];
+        if (hist) hist[it] = fit;
+        if (it > 0 && fit < prev_fit) osc += 1;
+        const double change = std::fabs(fit - prev_fit);
+        final_change = change;
+        if (it > 0 && change < o->tol) {
+          reason = TENRSO_REASON_FIT_TOLERANCE;
+          break;
+        }
+        if (osc > 5 && it > 10) {
+          reason = TENRSO_REASON_OSCILLATION;
+          break;
+        }
+        prev_fit = fit;
+        if (k >= o->max_iters) {
+          reason = TENRSO_REASON_MAX_ITERATIONS;
+          break;
+        }
+        if (!spec) {
+          if (time_up()) {
+            reason = TENRSO_REASON_TIME_LIMIT;
+            break;
+          }
+          TENRSO_TRY(enqueue_sweep(k, false));
+        }
+      }
     }
-    // fit bookkeeping in FP64 (the fit itself is an FP64 reduction on the device)
-    fit = hs[SC_FIT];
-    if (hist) hist[iter] = fit;
-    if (iter > 0 && fit < prev_fit) osc += 1;
-    const double change = std::fabs(fit - prev_fit);
-    final_change = change;
-    if (iter > 0 && change < o->tol) {
-      reason = TENRSO_REASON_FIT_TOLERANCE;
-      break;
-    }
-    if (osc > 5 && iter > 10) {
-      reason = TENRSO_REASON_OSCILLATION;
-      break;
-    }
-    prev_fit = fit;
   }
-  TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));  // drains a speculative sweep
+  if (status != TENRSO_OK) return status;
+  // the result of the last committed sweep (iters - 1) is in set iters % 2
+  if ((iters & 1) == 1) {
+    for (int k = 0; k < nd; ++k)
+      TENRSO_HIP_TRY(hipMemcpyAsync(F[k], set[1][k], (size_t)shape[k] * R * sizeof(float), hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+    TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
   if (res) {
     res->iters = iters;
     res->reason = reason;
@@ -211,9 +297,9 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
     Xt = pt;
     return TENRSO_OK;
   };
-  pb.mttkrp = [&](int mode, const float* const* Fc, MttkrpResult* mr) {
+  pb.mttkrp = [&](int mode, const float* const* Fc, const AlsJob* job, MttkrpResult* mr) {
     return mttkrp_dense(ctx, bf16 ? TENRSO_PREC_BF16 : TENRSO_PREC_FP32, Xk, Xt, nd, shape, Fc, R, mode,
-                        MttkrpOut::PARTIAL, nullptr, mr);
+                        MttkrpOut::PARTIAL, nullptr, mr, job);
   };
   return cp_als_run(ctx, pb, F, o, res, hist);
 }

@@ -1,18 +1,17 @@
-// cp_linalg.hip -- FP64 normal-equation kernels of the ALS step on gfx950.
+// cp_linalg.hip -- FP64 kernels of the ALS step on gfx950 besides the MTTKRP.
 //
-//   Gram-Hadamard  Gamma_n = hadamard_{k != n} A_k^T A_k      (cp/helpers.rs:409-452)
-//   solve          A_n[i,:] = M[i,:] Gamma_n^{-1} for every row (cp/helpers.rs:455-495)
-//   fit            1 - sqrt(max(0, |X|^2 + |Xhat|^2 - 2<X,Xhat>)) / |X|   (cp/helpers.rs:498-579)
+//   Gram           G = A^T A                                   (cp/helpers.rs:409-452)
+//   ALS job        Gamma_n^{-1} and the fit as one workgroup (als_job.hpp)
+//   row solve      A_n[i,:] = M[i,:] Gamma_n^{-1} for every row (cp/helpers.rs:455-495)
 //
 // All of it runs in FP64 on the device; the factors stay FP32.  Every reduction
 // has a fixed order (per-block partials summed in block order), so results are
-// run-to-run deterministic.  The R x R inverse is an in-LDS Gauss-Jordan
-// elimination without pivoting, valid for the symmetric positive definite
-// Gamma_n; a non-positive pivot triggers the reference's ridge retry
-// (Gamma += T::epsilon() * 10R * I with T = f32, cp/helpers.rs:477-490).
+// run-to-run deterministic.
 #include <cfloat>
 
+#include "als_job.hpp"
 #include "cp_linalg.hpp"
+#include "mttkrp.hpp"
 
 namespace tenrso {
 
@@ -128,145 +127,18 @@ int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G, hipStr
   return TENRSO_OK;
 }
 
-// ---------------------------------------------------------------- normal equations
-// One workgroup: Gamma = hadamard_{k != skip} G_k (ascending k, from ones) + reg,
-// padded to 64 x 64 with the identity, inverted in place by Gauss-Jordan.
-// Thread t owns column c = t % 64 and rows rg + 4m (m < 16) in registers; each
-// pivot step publishes the pivot row and column through double-buffered LDS,
-// so one barrier per step suffices.  The output Winv is 64 x 64 (stride 64);
-// entries outside R x R are the identity and never used.
-constexpr int NE = 64;
-
-struct NormalEqArgs {
-  const double* G;
-  int nd, skip;
-  int R;
-  double l2;
-  int tik_order;
-  double tik_lambda;
-  double* Winv;
-  double* scal;
-};
-
-__device__ __forceinline__ double gamma_entry(const NormalEqArgs& a, int i, int j, double ridge) {
-  const int R = a.R;
-  if (i >= R || j >= R) return i == j ? 1.0 : 0.0;
-  double v = 1.0;
-  for (int k = 0; k < a.nd; ++k)
-    if (k != a.skip) v = v * a.G[(i64)k * R * R + i * R + j];
-  if (a.tik_order > 0) {
-    // lambda * D^T D for first / second differences (cp/helpers.rs:630-702)
-    const double lam = a.tik_lambda;
-    const int n = R;
-    if (a.tik_order == 1) {
-      if (n >= 2) {
-        if (i == j) v += lam * ((i == 0 || i == n - 1) ? 1.0 : 2.0);
-        else if (j == i + 1 || i == j + 1) v -= lam;
-      }
-    } else {
-      if (n >= 3) {
-        if (i == j) v += lam * ((i == 0 || i == n - 1) ? 1.0 : ((i == 1 || i == n - 2) ? 5.0 : 6.0));
-        else if (j == i + 1 || i == j + 1) {
-          const int lo = i < j ? i : j;
-          v += lam * ((lo == 0 || lo == n - 2) ? -2.0 : -4.0);
-        } else if (j == i + 2 || i == j + 2) {
-          v += lam;
-        }
-      } else if (i == j) {
-        v += lam;
-      }
-    }
-  } else if (a.l2 > 0.0 && i == j) {
-    v += a.l2;
-  }
-  if (i == j) v += ridge;
-  return v;
+// ---------------------------------------------------------------- ALS job
+__global__ __launch_bounds__(JOB_THREADS) void als_job_kernel(AlsJob job) {
+  __shared__ __attribute__((aligned(16))) double lds[JOB_LDS_BYTES / 8];
+  als_job_run(job, lds);
 }
 
-// FP64 reciprocal: hardware estimate refined by two Newton steps.
-__device__ __forceinline__ double recip(double p) {
-  double r = __builtin_amdgcn_rcp(p);
-  r = fma(r, fma(-p, r, 1.0), r);
-  r = fma(r, fma(-p, r, 1.0), r);
-  return r;
-}
-
-// In-place Gauss-Jordan inverse of the 64 x 64 matrix held as av[m] = A[rg+4m][c]
-// (classic in-place form: after step j, column j holds column j of the inverse).
-// Step j:
-//   * the wave owning row j (rg == j % 4) reads the pivot p = A[j][j] from lane j
-//     (readlane), publishes 1/p and the scaled pivot row prow[c] = A[j][c] / p;
-//   * the four threads with c == j publish the pivot column pcol[i] = A[i][j],
-//     except pcol[j] = p - 1, and reset their own entries to the identity
-//     column (1 on row j, 0 elsewhere);
-//   * one barrier; then every entry does a single FMA
-//       A[i][c] -= pcol[i] * prow[c]
-//     which yields A[i][c] - A[i][j] A[j][c] / p off the pivot row and column,
-//     A[j][c] / p on the pivot row, -A[i][j] / p on the pivot column and 1/p on
-//     the pivot.  A bad pivot (<= 0 or not finite) clears `ok` (no branch).
-__device__ bool gj_invert(double (&av)[16], int c, int rg, double* prow, double* pcol, double* pinv) {
-  bool ok = true;
-  for (int j = 0; j < NE; ++j) {
-    const int b = j & 1;
-    double p = 0.0;
-    if (rg == (j & 3)) {
-      // pv = av[j / 4] without a dynamic register index
-      double pv = 0.0;
-#pragma unroll
-      for (int m = 0; m < 16; ++m) pv = (m == (j >> 2)) ? av[m] : pv;
-      const int lo = __builtin_amdgcn_readlane(__double2loint(pv), j);
-      const int hi = __builtin_amdgcn_readlane(__double2hiint(pv), j);
-      p = __hiloint2double(hi, lo);
-      const double ip = recip(p);
-      prow[b * NE + c] = (c == j) ? ip : pv * ip;
-      if (c == 0) pinv[b] = (p > 0.0 && isfinite(p)) ? ip : __longlong_as_double(0x7ff8000000000000LL);
-    }
-    if (c == j) {
-#pragma unroll
-      for (int m = 0; m < 16; ++m) {
-        const bool diag = (rg + 4 * m) == j;
-        pcol[b * NE + rg + 4 * m] = diag ? av[m] - 1.0 : av[m];
-        av[m] = diag ? 1.0 : 0.0;
-      }
-    }
-    __syncthreads();
-    ok = ok && !isnan(pinv[b]);
-    const double ajc = prow[b * NE + c];
-#pragma unroll
-    for (int m = 0; m < 16; ++m) av[m] = fma(-pcol[b * NE + rg + 4 * m], ajc, av[m]);
-  }
-  __syncthreads();
-  return ok;
-}
-
-__global__ __launch_bounds__(256) void normal_eq_kernel(NormalEqArgs a) {
-  __shared__ double prow[2 * NE], pcol[2 * NE], pinv[2];
-  const int c = threadIdx.x & 63, rg = threadIdx.x >> 6;
-  double av[16];
-#pragma unroll
-  for (int m = 0; m < 16; ++m) av[m] = gamma_entry(a, rg + 4 * m, c, 0.0);
-  int path = 0;
-  bool ok = gj_invert(av, c, rg, prow, pcol, pinv);
-  if (!ok) {
-    path = 1;
-#pragma unroll
-    for (int m = 0; m < 16; ++m) av[m] = gamma_entry(a, rg + 4 * m, c, (double)FLT_EPSILON * (double)(10 * a.R));
-    ok = gj_invert(av, c, rg, prow, pcol, pinv);
-    if (!ok) path = 2;
-  }
-#pragma unroll
-  for (int m = 0; m < 16; ++m) a.Winv[(rg + 4 * m) * NE + c] = ok ? av[m] : 0.0;
-  if (threadIdx.x == 0 && a.scal) a.scal[SC_STATUS] = fmax(a.scal[SC_STATUS], (double)path);
-}
-
-int normal_eq_inverse(tenrso_ctx* ctx, const double* G, int nd, int skip, i64 R, const GramReg& reg,
-                      double* Winv, double* scal, hipStream_t stream) {
-  if (R > SOLVE_RMAX) {
-    set_error("rank %lld exceeds the on-device normal-equation solve bound %d", (long long)R, SOLVE_RMAX);
+int als_job_launch(tenrso_ctx* ctx, const AlsJob& job) {
+  if (job.R > JOB_NE) {
+    set_error("rank %d exceeds the on-device normal-equation solve bound %d", job.R, JOB_NE);
     return TENRSO_ERR_UNSUPPORTED;
   }
-  NormalEqArgs a{G, nd, skip, (int)R, reg.l2, reg.tikhonov_order, reg.tikhonov_lambda, Winv, scal};
-  hipLaunchKernelGGL(normal_eq_kernel, dim3(1), dim3(256), 0, stream ? stream : ctx->stream, a);
+  hipLaunchKernelGGL(als_job_kernel, dim3(1), dim3(JOB_THREADS), 0, ctx->stream, job);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -277,6 +149,7 @@ int normal_eq_inverse(tenrso_ctx* ctx, const double* G, int nd, int skip, i64 R,
 // column (all loads in flight, FP64), the quarters are added in fixed order,
 // then A[i, c] = epi(sum_r M[i, r] Winv[r, c]) (the r-sum split over the same
 // four thread groups) and the block's <M, A> partial is written.
+constexpr int NE = JOB_NE;  // stride of Winv
 constexpr int SOLVE_RB = 1;
 
 __global__ __launch_bounds__(256) void solve_rows_kernel(const float* __restrict__ part, int splits,
@@ -379,39 +252,6 @@ int inner_rows(tenrso_ctx* ctx, const double* M, const float* A, i64 rows, i64 R
 }
 
 // ---------------------------------------------------------------- fit
-__global__ __launch_bounds__(256) void fit_kernel(const double* __restrict__ G, int nd, int R,
-                                                  const double* __restrict__ ip_part, int nblk,
-                                                  double* __restrict__ scal) {
-  __shared__ double red[4];
-  // ||Xhat||^2 = sum_{r,s} prod_k G_k[r,s]   (cp/helpers.rs:533-555)
-  double rec = 0.0;
-  for (int e = threadIdx.x; e < R * R; e += 256) {
-    double cross = 1.0;
-    for (int k = 0; k < nd; ++k) cross = cross * G[(i64)k * R * R + e];
-    rec += cross;
-  }
-  const double recon = block_sum_256(rec, red);
-  double ipv = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += 256) ipv += ip_part[b];
-  const double ip = block_sum_256(ipv, red);
-  if (threadIdx.x == 0) {
-    const double nx = scal[SC_NORM_X];
-    const double err_sq = nx + recon - 2.0 * ip;
-    const double err = sqrt(err_sq > 0.0 ? err_sq : 0.0);
-    double fit = 1.0 - err / sqrt(nx);
-    fit = fit < 0.0 ? 0.0 : (fit > 1.0 ? 1.0 : fit);
-    scal[SC_FIT] = fit;
-    scal[SC_IP] = ip;
-    scal[SC_RECON] = recon;
-  }
-}
-
-int fit_from_grams(tenrso_ctx* ctx, const double* G, int nd, i64 R, const double* ip_part, int nblk, double* scal) {
-  hipLaunchKernelGGL(fit_kernel, dim3(1), dim3(256), 0, ctx->stream, G, nd, (int)R, ip_part, nblk, scal);
-  TENRSO_HIP_TRY(hipGetLastError());
-  return TENRSO_OK;
-}
-
 // ---------------------------------------------------------------- small utilities
 __global__ __launch_bounds__(256) void clamp_nonneg_kernel(float* __restrict__ A, i64 n) {
   const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;

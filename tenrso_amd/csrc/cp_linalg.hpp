@@ -25,8 +25,9 @@ enum Scalars : int {
   SC_FIT = 1,      // fit of the last evaluation
   SC_IP = 2,       // <X, Xhat>
   SC_RECON = 3,    // ||Xhat||^2
-  SC_STATUS = 4,   // solve path: 0 Cholesky-class, 1 ridge retry, 2 failure
+  SC_STATUS = 4,   // solve path of even sweeps: 0 plain, 1 ridge retry, 2 failure
   SC_MIN_X = 5,    // min(X)
+  SC_STATUS_ODD = 6,  // solve path of odd sweeps
   SC_COUNT = 8
 };
 
@@ -34,19 +35,12 @@ enum Scalars : int {
 int tensor_norm_min(tenrso_ctx* ctx, const float* X, i64 n, double* tmp, double* scal);
 // G = A^T A (FP64) for an fp32 rows x R factor, on `stream` (null: the context stream).
 int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G, hipStream_t stream = nullptr);
-// Winv = (hadamard_{k != skip} G_k + reg)^{-1}, 64 x 64 (stride 64, identity padding);
-// G: [nd][R][R].  Writes the path taken into scal[SC_STATUS] (max with the previous
-// value).  Runs on `stream` (null: the context stream).
-int normal_eq_inverse(tenrso_ctx* ctx, const double* G, int nd, int skip, i64 R, const GramReg& reg,
-                      double* Winv, double* scal, hipStream_t stream);
 // A[i,:] = epi(M[i,:] Winv), M = sum of the `splits` fp32 partials [splits][rows][R]
 // (or the fp64 M when part is null); ip_part[blk] = sum_i M[i,:].A[i,:] (FP64).
 int solve_rows(tenrso_ctx* ctx, const float* part, int splits, const double* M, i64 rows, i64 R,
                const double* Winv, const RowEpilogue& epi, float* A, double* ip_part, int* nblk_out);
 // ip_part[blk] = sum_{i in blk} M[i,:] . A[i,:]
 int inner_rows(tenrso_ctx* ctx, const double* M, const float* A, i64 rows, i64 R, double* ip_part, int* nblk_out);
-// fit from the Grams, the ip partials and ||X||^2 (cp/helpers.rs:513-555).
-int fit_from_grams(tenrso_ctx* ctx, const double* G, int nd, i64 R, const double* ip_part, int nblk, double* scal);
 // x = max(x, 0) on an fp32 array.
 int clamp_nonneg(tenrso_ctx* ctx, float* A, i64 n);
 // fill an fp64 array

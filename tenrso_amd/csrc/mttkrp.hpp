@@ -4,6 +4,8 @@
 
 namespace tenrso {
 
+struct AlsJob;
+
 constexpr int MAXD = 8;
 
 // Output form of a dense MTTKRP launch.
@@ -21,8 +23,14 @@ struct MttkrpResult {
 // X: fp32 tensor (prec FP32) or bf16 tensor (prec BF16, uint16 storage).  For BF16
 // and mode == ndim-1 the caller may pass X_t, the tensor with the last mode moved
 // to the front (see make_bf16_copies); if X_t is null the launcher builds it.
+// job (optional): the ALS linear algebra of this mode; it runs as workgroup 0 of the
+// MTTKRP launch (or, where it cannot, as its own kernel just before it).
 int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int nd, const i64* shape,
-                 const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res);
+                 const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res,
+                 const AlsJob* job = nullptr);
+
+// The ALS job as its own one-workgroup kernel on the context stream.
+int als_job_launch(tenrso_ctx* ctx, const AlsJob& job);
 
 // Sum split partials [splits, n] in FP64 into out (fp32 or fp64).
 int reduce_partials(tenrso_ctx* ctx, const float* partial, int splits, i64 n, float* out32, double* out64);

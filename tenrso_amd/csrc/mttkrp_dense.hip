@@ -27,6 +27,7 @@
 //     atomics: results are run-to-run deterministic.
 #include <algorithm>
 
+#include "als_job.hpp"
 #include "mttkrp.hpp"
 
 namespace tenrso {
@@ -48,6 +49,7 @@ struct KernArgs {
   int slow_splits;    // splits of the slow range; grid.y = nwin * slow_splits
   int r0;             // first rank column of this launch
   int rcount;         // rank columns handled by the launch (<= 16*NT)
+  int rtiles;         // row tiles; workgroup tile -> (tile % rtiles, split = tile / rtiles)
 };
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
@@ -146,14 +148,24 @@ __device__ __forceinline__ void fold_slow(floatx4 (&acc)[MT][NT], floatx4 (&T)[M
 // ---------------------------------------------------------------------------
 template <int NT, int WAVES, int MT, bool CASE_B, int KWMAX>
 __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __restrict__ X, KernArgs a,
-                                                              float* __restrict__ partial) {
+                                                              float* __restrict__ partial, AlsJob job, int has_job) {
   constexpr int KB = 32, PITCH = KWMAX + 4, RT = 16 * NT, NTH = WAVES * 64, ROWS = 16 * MT;
   constexpr int LDS_WIN = RT * PITCH, LDS_RED = (WAVES - 1) * MT * NT * 4 * 64;
-  __shared__ __attribute__((aligned(16))) float smem[LDS_WIN > LDS_RED ? LDS_WIN : LDS_RED];
+  constexpr int LDS_F = LDS_WIN > LDS_RED ? LDS_WIN : LDS_RED;
+  constexpr int LDS_FLOATS = LDS_F * 4 > JOB_LDS_BYTES ? LDS_F : JOB_LDS_BYTES / 4;
+  __shared__ __attribute__((aligned(16))) float smem[LDS_FLOATS];
+  static_assert(NTH == JOB_THREADS, "the ALS job needs a 256-thread workgroup");
+  // workgroup 0 of a launch with a job runs the ALS linear algebra of the mode
+  if (has_job && blockIdx.x == 0) {
+    als_job_run(job, reinterpret_cast<double*>(smem));
+    return;
+  }
+  const int tile = (int)blockIdx.x - (has_job ? 1 : 0);
+  const int xt = tile % a.rtiles;
   float* sF = smem;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = lane >> 4, c = lane & 15;
-  const int split = blockIdx.y;
+  const int split = tile / a.rtiles;
   const int w = split % a.nwin;
   const int ss = split / a.nwin;
   const i64 s_lo = (i64)ss * a.n_slow / a.slow_splits;
@@ -161,7 +173,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
   const int nkb = a.kw / KB;
   const i64 my_slow = (s_hi - s_lo > wave) ? (s_hi - s_lo - wave + WAVES - 1) / WAVES : 0;
   const i64 T = my_slow * nkb;
-  const i64 row_base = (i64)blockIdx.x * ROWS;
+  const i64 row_base = (i64)xt * ROWS;
 
   stage_fast_window<RT, PITCH, NTH>(sF, a, w);
 
@@ -300,16 +312,24 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
 // ---------------------------------------------------------------------------
 template <int NT, int WAVES, int MT, int KS, int KWMAX>
 __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* __restrict__ X, KernArgs a,
-                                                               float* __restrict__ partial) {
+                                                               float* __restrict__ partial, AlsJob job, int has_job) {
   constexpr int KB = 32 * KS, PITCH = KWMAX + 8, RT = 16 * NT, NTH = WAVES * 64, ROWS = 16 * MT;
   constexpr int LDS_WIN_U16 = 2 * RT * PITCH, LDS_RED_F = (WAVES - 1) * MT * NT * 4 * 64;
-  constexpr int LDS_BYTES = (LDS_WIN_U16 * 2 > LDS_RED_F * 4) ? LDS_WIN_U16 * 2 : LDS_RED_F * 4;
+  constexpr int LDS_B = (LDS_WIN_U16 * 2 > LDS_RED_F * 4) ? LDS_WIN_U16 * 2 : LDS_RED_F * 4;
+  constexpr int LDS_BYTES = LDS_B > JOB_LDS_BYTES ? LDS_B : JOB_LDS_BYTES;
   __shared__ __attribute__((aligned(16))) float smem[LDS_BYTES / 4];
+  static_assert(NTH == JOB_THREADS, "the ALS job needs a 256-thread workgroup");
+  if (has_job && blockIdx.x == 0) {
+    als_job_run(job, reinterpret_cast<double*>(smem));
+    return;
+  }
+  const int tile = (int)blockIdx.x - (has_job ? 1 : 0);
+  const int xt = tile % a.rtiles;
   uint16_t* sHi = reinterpret_cast<uint16_t*>(smem);
   uint16_t* sLo = sHi + RT * PITCH;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int g = lane >> 4, c = lane & 15;
-  const int split = blockIdx.y;
+  const int split = tile / a.rtiles;
   const int w = split % a.nwin;
   const int ss = split / a.nwin;
   const i64 s_lo = (i64)ss * a.n_slow / a.slow_splits;
@@ -317,7 +337,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
   const int nkb = a.kw / KB;
   const i64 my_slow = (s_hi - s_lo > wave) ? (s_hi - s_lo - wave + WAVES - 1) / WAVES : 0;
   const i64 T = my_slow * nkb;
-  const i64 row_base = (i64)blockIdx.x * ROWS;
+  const i64 row_base = (i64)xt * ROWS;
 
   stage_fast_window_bf16x2<RT, PITCH, NTH>(sHi, sLo, a, w);
 
@@ -618,7 +638,8 @@ Plan make_plan(int nd, const i64* shape, int mode) {
 }  // namespace
 
 int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int nd, const i64* shape,
-                 const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res) {
+                 const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res,
+                 const AlsJob* job) {
   Plan pl = make_plan(nd, shape, mode);
   const bool bf16 = prec == TENRSO_PREC_BF16;
   const float* Fv[MAXD];
@@ -703,8 +724,10 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     a.nwin = (int)(pl.fast_size / kw);
     a.n_slow = pl.J / pl.fast_size;
     a.qs = caseB ? 1 : pl.Q / pl.fast_size;
-    // about wg_per_cu workgroups per CU; each workgroup keeps >= WAVES slow indices
-    const i64 target = (i64)(bf16 ? ctx->wg_per_cu_bf16 : ctx->wg_per_cu_f32) * ctx->num_cus;
+    // about wg_per_cu workgroups per CU (one slot fewer when the ALS job shares the
+    // launch, so every workgroup is resident at once); each keeps >= WAVES slow indices
+    const bool fuse_job = job != nullptr && rchunks == 1;
+    const i64 target = (i64)(bf16 ? ctx->wg_per_cu_bf16 : ctx->wg_per_cu_f32) * ctx->num_cus - (fuse_job ? 1 : 0);
     i64 ssp = std::max<i64>(1, target / std::max<i64>(1, tiles) / a.nwin);
     ssp = std::min<i64>(ssp, a.n_slow >= WAVES ? a.n_slow / WAVES : 1);
     ssp = std::min<i64>(ssp, 65535 / a.nwin);
@@ -713,14 +736,23 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     void* pbuf = nullptr;
     TENRSO_TRY(ctx_buffer(ctx, SLOT_PARTIAL, (size_t)splits * nout * sizeof(float), &pbuf));
     partial = (float*)pbuf;
+    a.rtiles = (int)rtiles;
+    if (job && !fuse_job) TENRSO_TRY(als_job_launch(ctx, *job));
     const dim3 block(WAVES * 64);
-    const dim3 grid((unsigned)rtiles, splits, 1);
-#define TENRSO_F32(NTV, MTV, CB) \
-  hipLaunchKernelGGL((mttkrp_f32_mfma<NTV, WAVES, MTV, CB, KWMAX>), grid, block, 0, ctx->stream, Xf, a, partial)
-#define TENRSO_B16(NTV, KSV) \
-  hipLaunchKernelGGL((mttkrp_bf16_mfma<NTV, WAVES, 2, KSV, KWMAX>), grid, block, 0, ctx->stream, Xb, a, partial)
-    auto launch_main = [&]() -> int {
+    AlsJob jarg{};
+    int has_job = 0;
+    dim3 grid;
+#define TENRSO_F32(NTV, MTV, CB)                                                                                   \
+  hipLaunchKernelGGL((mttkrp_f32_mfma<NTV, WAVES, MTV, CB, KWMAX>), grid, block, 0, ctx->stream, Xf, a, partial, \
+                     jarg, has_job)
+#define TENRSO_B16(NTV, KSV)                                                                                       \
+  hipLaunchKernelGGL((mttkrp_bf16_mfma<NTV, WAVES, 2, KSV, KWMAX>), grid, block, 0, ctx->stream, Xb, a, partial, \
+                     jarg, has_job)
+    auto launch_main = [&](bool with_job) -> int {
       for (int rc = 0; rc < rchunks; ++rc) {
+        has_job = (with_job && fuse_job && rc == 0) ? 1 : 0;
+        if (has_job) jarg = *job;
+        grid = dim3((unsigned)(rtiles * splits + has_job), 1, 1);
         a.r0 = rc * RT;
         a.rcount = (int)std::min<i64>(RT, R - a.r0);
         if (!bf16) {
@@ -758,18 +790,19 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
 #undef TENRSO_B16
     if (ctx->profile_reps > 0) {
       // profiling: the main kernel alone, `profile_reps` times between two events
-      TENRSO_TRY(launch_main());
+      TENRSO_TRY(launch_main(true));
       TENRSO_HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-      for (int rep = 0; rep < ctx->profile_reps; ++rep) TENRSO_TRY(launch_main());
+      for (int rep = 0; rep < ctx->profile_reps; ++rep) TENRSO_TRY(launch_main(false));
       TENRSO_HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
       TENRSO_HIP_TRY(hipEventSynchronize(ctx->ev1));
       float ms = 0.f;
       TENRSO_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
       ctx->last_kernel_ms = (double)ms / ctx->profile_reps;
     } else {
-      TENRSO_TRY(launch_main());
+      TENRSO_TRY(launch_main(true));
     }
   } else {
+    if (job) TENRSO_TRY(als_job_launch(ctx, *job));
     GenericArgs ga{};
     ga.nd = nd;
     ga.mode = mode_use;

@@ -67,7 +67,7 @@ def test_csf_reference_kats():
 
 
 def test_csf_roundtrip_multi_nonzero_fibers():
-    # the case the reference's iterator mis-decodes (fibers with >1 nonzero, SURVEY:
+    # the case the reference iterator mis-decodes (fibers with >1 nonzero; in the
     # csf.rs doc example, nonzero (1,2,3) came back as (0,2,3)); here it round-trips
     rng = np.random.default_rng(0)
     shape = (6, 5, 7, 3)
