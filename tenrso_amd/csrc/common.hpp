@@ -59,8 +59,8 @@ struct tenrso_ctx {
   hipEvent_t ev_fit[2] = {nullptr, nullptr};  // "fit of sweep k is in pinned[k % 2]" markers
   double last_kernel_ms = 0.0;
   int profile_reps = 0;               // > 0: mttkrp_dense times its main kernel
-  int wg_per_cu_f32 = 2;              // MTTKRP grid sizing (TENRSO_WG_PER_CU_F32 / _BF16)
-  int wg_per_cu_bf16 = 2;
+  int wg_per_cu_f32 = 0;              // MTTKRP workgroups per CU; 0: the kernel's occupancy
+  int wg_per_cu_bf16 = 0;             // (TENRSO_WG_PER_CU_F32 / _BF16 override)
   std::vector<tenrso::DevBuf> bufs;  // indexed by slot id
   void* pinned = nullptr;            // small pinned host buffer for scalar readback
 };

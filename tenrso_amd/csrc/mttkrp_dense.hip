@@ -26,6 +26,8 @@
 //     fixed order (FP64) by reduce_partials or by the ALS solve kernel.  No float
 //     atomics: results are run-to-run deterministic.
 #include <algorithm>
+#include <tuple>
+#include <type_traits>
 
 #include "als_job.hpp"
 #include "mttkrp.hpp"
@@ -611,6 +613,13 @@ int transpose_last_to_front_u16(tenrso_ctx* ctx, const uint16_t* src, int nd, co
 
 namespace {
 
+template <typename F>
+struct KernelArgs;
+template <typename... A>
+struct KernelArgs<void (*)(A...)> {
+  using type = std::tuple<A...>;
+};
+
 struct Plan {
   i64 P = 1, I = 1, Q = 1, J = 1;
   int n_oth = 0;
@@ -724,10 +733,47 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     a.nwin = (int)(pl.fast_size / kw);
     a.n_slow = pl.J / pl.fast_size;
     a.qs = caseB ? 1 : pl.Q / pl.fast_size;
-    // about wg_per_cu workgroups per CU (one slot fewer when the ALS job shares the
-    // launch, so every workgroup is resident at once); each keeps >= WAVES slow indices
+    a.rtiles = (int)rtiles;
+    // the kernel instance for this shape: calls f(kernel) with its function pointer
+    auto with_kernel = [&](auto&& f) {
+      if (!bf16) {
+        if (!caseB) {
+          if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, false, KWMAX>);
+          if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 2, false, KWMAX>);
+          return f(mttkrp_f32_mfma<4, WAVES, 2, false, KWMAX>);
+        }
+        if (MT == 2) {
+          if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, true, KWMAX>);
+          if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 2, true, KWMAX>);
+          return f(mttkrp_f32_mfma<4, WAVES, 2, true, KWMAX>);
+        }
+        if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 1, true, KWMAX>);
+        if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 1, true, KWMAX>);
+        return f(mttkrp_f32_mfma<4, WAVES, 1, true, KWMAX>);
+      }
+      if (KB == 64) {
+        if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES, 2, 2, KWMAX>);
+        if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES, 2, 2, KWMAX>);
+        return f(mttkrp_bf16_mfma<4, WAVES, 2, 2, KWMAX>);
+      }
+      if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES, 2, 1, KWMAX>);
+      if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES, 2, 1, KWMAX>);
+      return f(mttkrp_bf16_mfma<4, WAVES, 2, 1, KWMAX>);
+    };
+    // workgroups per CU: what the instance can keep resident (registers, LDS),
+    // unless TENRSO_WG_PER_CU_* overrides it; one slot fewer when the ALS job
+    // shares the launch, so every workgroup is resident at once
+    int wpc = bf16 ? ctx->wg_per_cu_bf16 : ctx->wg_per_cu_f32;
+    if (wpc <= 0) {
+      int nb = 0;
+      TENRSO_HIP_TRY(with_kernel([&](auto kern) {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, WAVES * 64, 0);
+      }));
+      wpc = std::max(1, nb);
+    }
     const bool fuse_job = job != nullptr && rchunks == 1;
-    const i64 target = (i64)(bf16 ? ctx->wg_per_cu_bf16 : ctx->wg_per_cu_f32) * ctx->num_cus - (fuse_job ? 1 : 0);
+    const i64 target = (i64)wpc * ctx->num_cus - (fuse_job ? 1 : 0);
+    // each workgroup keeps >= WAVES slow indices
     i64 ssp = std::max<i64>(1, target / std::max<i64>(1, tiles) / a.nwin);
     ssp = std::min<i64>(ssp, a.n_slow >= WAVES ? a.n_slow / WAVES : 1);
     ssp = std::min<i64>(ssp, 65535 / a.nwin);
@@ -736,58 +782,26 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     void* pbuf = nullptr;
     TENRSO_TRY(ctx_buffer(ctx, SLOT_PARTIAL, (size_t)splits * nout * sizeof(float), &pbuf));
     partial = (float*)pbuf;
-    a.rtiles = (int)rtiles;
     if (job && !fuse_job) TENRSO_TRY(als_job_launch(ctx, *job));
     const dim3 block(WAVES * 64);
-    AlsJob jarg{};
-    int has_job = 0;
-    dim3 grid;
-#define TENRSO_F32(NTV, MTV, CB)                                                                                   \
-  hipLaunchKernelGGL((mttkrp_f32_mfma<NTV, WAVES, MTV, CB, KWMAX>), grid, block, 0, ctx->stream, Xf, a, partial, \
-                     jarg, has_job)
-#define TENRSO_B16(NTV, KSV)                                                                                       \
-  hipLaunchKernelGGL((mttkrp_bf16_mfma<NTV, WAVES, 2, KSV, KWMAX>), grid, block, 0, ctx->stream, Xb, a, partial, \
-                     jarg, has_job)
     auto launch_main = [&](bool with_job) -> int {
       for (int rc = 0; rc < rchunks; ++rc) {
-        has_job = (with_job && fuse_job && rc == 0) ? 1 : 0;
+        const int has_job = (with_job && fuse_job && rc == 0) ? 1 : 0;
+        AlsJob jarg{};
         if (has_job) jarg = *job;
-        grid = dim3((unsigned)(rtiles * splits + has_job), 1, 1);
+        const dim3 grid((unsigned)(rtiles * splits + has_job), 1, 1);
         a.r0 = rc * RT;
         a.rcount = (int)std::min<i64>(RT, R - a.r0);
-        if (!bf16) {
-          const float* Xf = (const float*)Xuse;
-          if (!caseB) {
-            if (NT == 1) TENRSO_F32(1, 2, false);
-            else if (NT == 2) TENRSO_F32(2, 2, false);
-            else TENRSO_F32(4, 2, false);
-          } else if (MT == 2) {
-            if (NT == 1) TENRSO_F32(1, 2, true);
-            else if (NT == 2) TENRSO_F32(2, 2, true);
-            else TENRSO_F32(4, 2, true);
-          } else {
-            if (NT == 1) TENRSO_F32(1, 1, true);
-            else if (NT == 2) TENRSO_F32(2, 1, true);
-            else TENRSO_F32(4, 1, true);
-          }
-        } else {
-          const uint16_t* Xb = (const uint16_t*)Xuse;
-          if (KB == 64) {
-            if (NT == 1) TENRSO_B16(1, 2);
-            else if (NT == 2) TENRSO_B16(2, 2);
-            else TENRSO_B16(4, 2);
-          } else {
-            if (NT == 1) TENRSO_B16(1, 1);
-            else if (NT == 2) TENRSO_B16(2, 1);
-            else TENRSO_B16(4, 1);
-          }
-        }
+        with_kernel([&](auto kern) {
+          using XT = std::remove_const_t<std::remove_pointer_t<
+              std::tuple_element_t<0, typename KernelArgs<decltype(kern)>::type>>>;
+          hipLaunchKernelGGL(kern, grid, block, 0, ctx->stream, (const XT*)Xuse, a, partial, jarg, has_job);
+          return 0;
+        });
         TENRSO_HIP_TRY(hipGetLastError());
       }
       return TENRSO_OK;
     };
-#undef TENRSO_F32
-#undef TENRSO_B16
     if (ctx->profile_reps > 0) {
       // profiling: the main kernel alone, `profile_reps` times between two events
       TENRSO_TRY(launch_main(true));
