@@ -13,9 +13,11 @@
 //     KR[j, r] = PRE[slow, r] * FAST[k, r], where FAST is the factor of the
 //     fastest other mode and PRE the forward KR of the remaining other modes (a
 //     factor itself for 3-way tensors).  A workgroup stages a window of FAST,
-//     transposed to [r][k], in LDS once; for each slow index it contracts
-//     T[i, r] = sum_k X[.., i, .., k] FAST[k, r] on MFMA and adds PRE[slow, r] T[i, r]
-//     into its accumulator, so no Khatri-Rao row is ever written anywhere.
+//     transposed to [r][k], in LDS once.  FP32 forms the Khatri-Rao operand in
+//     registers (FAST from LDS times PRE[slow, r]); BF16 contracts
+//     T[i, r] = sum_k X[.., i, .., k] FAST[k, r] per slow index and adds
+//     PRE[slow, r] T[i, r] into its accumulator.  No Khatri-Rao row is ever
+//     written to memory.
 //   * Rows i are the MFMA M dimension, rank columns r the N dimension, j the K
 //     dimension.  FP32 uses v_mfma_f32_16x16x4_f32; BF16 uses
 //     v_mfma_f32_16x16x32_bf16.  For q-contiguous X (mode < N-1, "case A") each
@@ -62,29 +64,51 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
 }
 
 // Stage FAST[w*kw .. w*kw+kw) (columns r0 .. r0+RT) into LDS transposed: s[r][kk].
-// Columns past rcount are zero, so they contribute nothing.
-template <int RT, int PITCH, int NTH>
+// Columns past rcount are zero, so they contribute nothing.  Every thread issues
+// all of its loads before its first LDS write (one memory round trip).
+template <int RT, int PITCH, int NTH, int KWMAX>
 __device__ __forceinline__ void stage_fast_window(float* __restrict__ s, const KernArgs& a, int w) {
+  constexpr int PER = (RT * KWMAX + NTH - 1) / NTH;
   const i64 k0 = (i64)w * a.kw;
-  for (int idx = threadIdx.x; idx < RT * a.kw; idx += NTH) {
+  const int n = RT * a.kw;
+  float v[PER];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int idx = threadIdx.x + e * NTH;
     const int r = idx % RT, kk = idx / RT;
-    s[r * PITCH + kk] = (r < a.rcount) ? a.fast[(k0 + kk) * a.R + a.r0 + r] : 0.0f;
+    v[e] = (idx < n && r < a.rcount) ? a.fast[(k0 + kk) * a.R + a.r0 + r] : 0.0f;
+  }
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int idx = threadIdx.x + e * NTH;
+    if (idx < n) s[(idx % RT) * PITCH + idx / RT] = v[e];
   }
 }
 
 // Same window as an exact bf16 pair: FAST = hi + lo with hi = bf16(FAST) and
 // lo = bf16(FAST - hi), so hi + lo carries 16 significant bits.
-template <int RT, int PITCH, int NTH>
+template <int RT, int PITCH, int NTH, int KWMAX>
 __device__ __forceinline__ void stage_fast_window_bf16x2(uint16_t* __restrict__ shi, uint16_t* __restrict__ slo,
                                                          const KernArgs& a, int w) {
+  constexpr int PER = (RT * KWMAX + NTH - 1) / NTH;
   const i64 k0 = (i64)w * a.kw;
-  for (int idx = threadIdx.x; idx < RT * a.kw; idx += NTH) {
+  const int n = RT * a.kw;
+  float v[PER];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int idx = threadIdx.x + e * NTH;
     const int r = idx % RT, kk = idx / RT;
-    const float v = (r < a.rcount) ? a.fast[(k0 + kk) * a.R + a.r0 + r] : 0.0f;
-    const uint16_t h = f32_to_bf16_rne(v);
-    const float vh = __uint_as_float((uint32_t)h << 16);
-    shi[r * PITCH + kk] = h;
-    slo[r * PITCH + kk] = f32_to_bf16_rne(v - vh);
+    v[e] = (idx < n && r < a.rcount) ? a.fast[(k0 + kk) * a.R + a.r0 + r] : 0.0f;
+  }
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int idx = threadIdx.x + e * NTH;
+    if (idx < n) {
+      const int o = (idx % RT) * PITCH + idx / RT;
+      const uint16_t h = f32_to_bf16_rne(v[e]);
+      shi[o] = h;
+      slo[o] = f32_to_bf16_rne(v[e] - __uint_as_float((uint32_t)h << 16));
+    }
   }
 }
 
@@ -134,11 +158,11 @@ __device__ __forceinline__ void fold_slow(floatx4 (&acc)[MT][NT], floatx4 (&T)[M
 //   WAVES waves share the LDS window and split the workgroup's slow range
 //   (wave k takes slow indices s_lo + k, s_lo + k + WAVES, ...); their
 //   accumulators are summed in LDS in fixed order at the end.
-//   For one slow index the contraction over the window is
-//     T[i, r] = sum_k X[.., i, .., k] * FAST[k, r]
-//   on MFMA (B operand = the FAST window, read as 4 consecutive k from LDS), and
-//   then acc[i, r] += PRE[slow, r] * T[i, r], which together is
-//   sum_j X_(n)[i, j] KR[j, r] with KR[j, r] = PRE[slow, r] * FAST[k, r].
+//   The B operand of the MFMAs is the Khatri-Rao row itself, formed on the fly:
+//   the FAST window is read as 4 consecutive k from LDS and scaled by
+//   PRE[slow, r] of the lane's column, KR[j, r] = PRE[slow, r] * FAST[k, r]
+//   (one fp32 rounding, as the reference's materialised KR).  The accumulators
+//   stay in the matrix-core registers for the whole loop.
 //   One block = KB = 32 consecutive j: 2 halves h x 4 lane groups g x 4 s, so
 //   MFMA step (h, s) of lane group g contracts j = j0 + 16h + 4g + s.
 //   Case A (q contiguous): lane (g, c) of m-tile mt holds row row_base+16mt+c and
@@ -146,7 +170,8 @@ __device__ __forceinline__ void fold_slow(floatx4 (&acc)[MT][NT], floatx4 (&T)[M
 //   Case B (i contiguous, last mode): rows are interleaved, lane c of m-tile mt
 //     holds row row_base + MT*c + mt, so one MT-wide vector load per j covers
 //     16*MT consecutive rows.
-//   X blocks are prefetched 3 blocks ahead in registers.
+//   X blocks are prefetched 3 blocks ahead in registers; the first loads are
+//   issued before the LDS window is staged.
 // ---------------------------------------------------------------------------
 template <int NT, int WAVES, int MT, bool CASE_B, int KWMAX>
 __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __restrict__ X, KernArgs a,
@@ -177,8 +202,6 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
   const i64 T = my_slow * nkb;
   const i64 row_base = (i64)xt * ROWS;
 
-  stage_fast_window<RT, PITCH, NTH>(sF, a, w);
-
   i64 rowc[MT];
   i64 rowv = 0;  // case B: first row of this lane's MT-vector
   if constexpr (!CASE_B) {
@@ -192,14 +215,11 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
     rowv = rr <= a.I - MT ? rr : a.I - MT;
   }
 
-  floatx4 acc[MT][NT], Tacc[MT][NT];
+  floatx4 acc[MT][NT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
-      Tacc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
-    }
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   struct Buf {
     floatx4 x[MT][2];
@@ -207,30 +227,46 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
     int kb;
   };
 
+  // per-lane element offsets, fixed for the whole loop; the per-block part of
+  // every address is wave-uniform
+  i64 loff[MT];
+  if constexpr (!CASE_B) {
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) loff[mt] = rowc[mt] * a.Q + 4 * g;
+  } else {
+    loff[0] = (i64)(4 * g) * a.I + rowv;
+  }
+  int poff[NT];
+  bool pok[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    poff[nt] = a.r0 + 16 * nt + c;
+    pok[nt] = a.pre != nullptr && 16 * nt + c < a.rcount;
+  }
+  const i64 IQ = a.I * a.Q;
+
   auto load = [&](i64 t, Buf& b) {
     if (t >= T) return;
     const int tq = (int)t / nkb;
     const int kb = (int)t - tq * nkb;
     const i64 slow = s_lo + wave + (i64)tq * WAVES;
     b.kb = kb;
+    const float* prow = a.pre + slow * a.R;
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const int r = 16 * nt + c;
-      b.pre[nt] = (a.pre && r < a.rcount) ? a.pre[slow * a.R + a.r0 + r] : 1.0f;
-    }
+    for (int nt = 0; nt < NT; ++nt) b.pre[nt] = pok[nt] ? prow[poff[nt]] : 1.0f;
     if constexpr (!CASE_B) {
       const int sl32 = (int)slow, qs32 = (int)a.qs;
-      const i64 p = sl32 / qs32;
-      const i64 q0 = (i64)(sl32 - (int)p * qs32) * a.fast_size + (i64)w * a.kw + kb * KB;
+      const int p = sl32 / qs32;
+      const float* ubase = X + (i64)p * IQ + (i64)(sl32 - p * qs32) * a.fast_size + (i64)w * a.kw + kb * KB;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        const float* base = X + (p * a.I + rowc[mt]) * a.Q + q0 + 4 * g;
+        const float* base = ubase + loff[mt];
         b.x[mt][0] = *reinterpret_cast<const floatx4*>(base);
         b.x[mt][1] = *reinterpret_cast<const floatx4*>(base + 16);
       }
     } else {
-      const i64 j0 = slow * a.fast_size + (i64)w * a.kw + kb * KB + 4 * g;
-      const float* base = X + j0 * a.I + rowv;
+      const i64 j0 = slow * a.fast_size + (i64)w * a.kw + kb * KB;
+      const float* base = X + j0 * a.I + loff[0];
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -258,22 +294,23 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
       floatx4 bv[NT];
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
-        bv[nt] = *reinterpret_cast<const floatx4*>(&sF[(16 * nt + c) * PITCH + b.kb * KB + 16 * h + 4 * g]);
+        bv[nt] = *reinterpret_cast<const floatx4*>(&sF[(16 * nt + c) * PITCH + b.kb * KB + 16 * h + 4 * g]) *
+                 b.pre[nt];
 #pragma unroll
       for (int sidx = 0; sidx < 4; ++sidx)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
           for (int mt = 0; mt < MT; ++mt)
-            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(b.x[mt][h][sidx], bv[nt][sidx], Tacc[mt][nt], 0, 0, 0);
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(b.x[mt][h][sidx], bv[nt][sidx], acc[mt][nt], 0, 0, 0);
     }
-    if (b.kb == nkb - 1) fold_slow<MT, NT>(acc, Tacc, b.pre);
   };
 
   Buf b0, b1, b2, b3;
   load(0, b0);
   load(1, b1);
   load(2, b2);
+  stage_fast_window<RT, PITCH, NTH, KWMAX>(sF, a, w);
   __syncthreads();  // LDS window staged
   for (i64 t = 0; t < T; t += 4) {
     load(t + 3, b3);
@@ -341,8 +378,6 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
   const i64 T = my_slow * nkb;
   const i64 row_base = (i64)xt * ROWS;
 
-  stage_fast_window_bf16x2<RT, PITCH, NTH>(sHi, sLo, a, w);
-
   i64 rowc[MT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) {
@@ -364,23 +399,35 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
     int kb;
   };
 
+  // per-lane element offsets, fixed for the whole loop; the per-block part of
+  // every address is wave-uniform
+  i64 loff[MT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) loff[mt] = rowc[mt] * a.Q + 8 * g;
+  int poff[NT];
+  bool pok[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    poff[nt] = a.r0 + 16 * nt + c;
+    pok[nt] = a.pre != nullptr && 16 * nt + c < a.rcount;
+  }
+  const i64 IQ = a.I * a.Q;
+
   auto load = [&](i64 t, Buf& b) {
     if (t >= T) return;
     const int tq = (int)t / nkb;
     const int kb = (int)t - tq * nkb;
     const i64 slow = s_lo + wave + (i64)tq * WAVES;
     b.kb = kb;
+    const float* prow = a.pre + slow * a.R;
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const int r = 16 * nt + c;
-      b.pre[nt] = (a.pre && r < a.rcount) ? a.pre[slow * a.R + a.r0 + r] : 1.0f;
-    }
+    for (int nt = 0; nt < NT; ++nt) b.pre[nt] = pok[nt] ? prow[poff[nt]] : 1.0f;
     const int sl32 = (int)slow, qs32 = (int)a.qs;
-    const i64 p = sl32 / qs32;
-    const i64 q0 = (i64)(sl32 - (int)p * qs32) * a.fast_size + (i64)w * a.kw + kb * KB;
+    const int p = sl32 / qs32;
+    const uint16_t* ubase = X + (i64)p * IQ + (i64)(sl32 - p * qs32) * a.fast_size + (i64)w * a.kw + kb * KB;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
-      const uint16_t* base = X + (p * a.I + rowc[mt]) * a.Q + q0 + 8 * g;
+      const uint16_t* base = ubase + loff[mt];
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) b.x[mt][ks] = *reinterpret_cast<const bf16x8*>(base + 32 * ks);
     }
@@ -408,7 +455,8 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
   load(0, b0);
   load(1, b1);
   load(2, b2);
-  __syncthreads();
+  stage_fast_window_bf16x2<RT, PITCH, NTH, KWMAX>(sHi, sLo, a, w);
+  __syncthreads();  // LDS window staged
   for (i64 t = 0; t < T; t += 4) {
     load(t + 3, b3);
     compute(t, b0);
