@@ -81,6 +81,133 @@ def cpu_baseline(X, init, gpu_fits, max_iters):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mttkrp_256cube_r64_traffic.md")
 
 
+def _dev_timed(fn):
+    """Run fn() once on torch's current stream, bracketed by events; returns (result, ms)."""
+    import torch
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    out = fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return out, e0.elapsed_time(e1)
+
+
+def extra_configs(with_cpu):
+    """The other workloads of the hot path, one GPU: the CPU-sized ALS case, the
+    ill-conditioned 512x512x128 case, the 4-way reverse-mode case and the 20M-nnz
+    sparse case (per-kernel times from HIP events)."""
+    import numpy as np
+    import torch
+    from tenrso_amd import _lib, ad, decomp, kernels, synth
+    from tenrso_amd.sparse import SparseTensor
+    out = {}
+    dev = torch.device("cuda:0")
+
+    # 64^3 rank 8, 20 iterations, tol 1e-4 (reference CPU-runnable case)
+    X, _, init = synth.dense_64_r8()
+    Xd = torch.from_numpy(X).to(dev)
+    Fd = [torch.from_numpy(f).to(dev) for f in init]
+    decomp.cp_als_with_init(Xd, Fd, 2, 0.0)
+    cp, ms = _dev_timed(lambda: decomp.cp_als_with_init(Xd, Fd, 20, 1e-4))
+    c1 = {"shape": [64, 64, 64], "rank": 8, "fit": cp.fit, "iters": cp.iters,
+          "reason": cp.convergence.reason.name, "gpu_s_per_iter": ms / 1e3 / max(cp.iters, 1)}
+    if with_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        _, info = oracle.cp_als(X, init, 20, 1e-4, dtype=np.float32)  # faithful f32 restatement, one thread
+        c1["cpu_port_s_per_iter"] = (time.perf_counter() - t0) / max(info["iters"], 1)
+        c1["cpu_port_fit"] = info["fit"]
+    out["dense_64_r8"] = c1
+
+    # 512x512x128 rank 64, congruence 0.9, noise 1e-4
+    X, _, _, init = synth.congruent(shape=(512, 512, 128))
+    Xd = torch.from_numpy(X).to(dev)
+    Fd = [torch.from_numpy(f).to(dev) for f in init]
+    ctx = _lib.torch_context(Xd)
+    Xb = kernels.to_bf16(Xd)
+    shape, R = X.shape, 64
+    modes = {}
+    for mode in range(3):
+        t32 = profile_kernel(ctx, _lib.PREC_FP32, Xd, shape, Fd, R, mode, 10)
+        t16 = profile_kernel(ctx, _lib.PREC_BF16, Xb, shape, Fd, R, mode, 10)
+        modes[str(mode)] = {"fp32_us": t32 * 1e3, "fp32_TFLOPs": mttkrp_flops(shape, R) / (t32 * 1e-3) / 1e12,
+                            "bf16_us": t16 * 1e3,
+                            "bf16_GBs": mttkrp_bytes(shape, R, mode, 2) / (t16 * 1e-3) / 1e9}
+    c3 = {"shape": list(shape), "rank": R, "mttkrp_per_mode": modes}
+    for prec in ("fp32", "bf16"):
+        decomp.cp_als_with_init(Xd, Fd, 2, 0.0, precision=prec)
+        cp, ms = _dev_timed(lambda: decomp.cp_als_with_init(Xd, Fd, 10, 0.0, precision=prec))
+        c3[prec] = {"s_per_iter": ms / 1e4, "fit": cp.fit}
+    out["congruent_512x512x128_r64"] = c3
+    del Xd, Xb
+
+    # 96^4 rank 32: MTTKRP per mode, CP-loss gradient, gradient-descent steps
+    X, _, _, init = synth.dense4_96_r32()
+    Xd = torch.from_numpy(X).to(dev)
+    Fd = [torch.from_numpy(f).to(dev) for f in init]
+    ctx = _lib.torch_context(Xd)
+    shape, R = X.shape, 32
+    Xb = kernels.to_bf16(Xd)
+    modes = {}
+    for mode in range(4):
+        t32 = profile_kernel(ctx, _lib.PREC_FP32, Xd, shape, Fd, R, mode, 10)
+        t16 = profile_kernel(ctx, _lib.PREC_BF16, Xb, shape, Fd, R, mode, 10)
+        modes[str(mode)] = {"fp32_us": t32 * 1e3, "fp32_GBs": mttkrp_bytes(shape, R, mode, 4) / (t32 * 1e-3) / 1e9,
+                            "bf16_us": t16 * 1e3, "bf16_GBs": mttkrp_bytes(shape, R, mode, 2) / (t16 * 1e-3) / 1e9}
+    del Xb
+    ad.cp_loss_and_grad(Xd, Fd)
+    (loss, _), ms_lg = _dev_timed(lambda: ad.cp_loss_and_grad(Xd, Fd))
+    Fg = [f.clone() for f in Fd]
+    hist, ms_gd = _dev_timed(lambda: ad.cp_gd(Xd, Fg, 20))
+    out["dense4_96_r32"] = {"shape": list(shape), "rank": R, "mttkrp_per_mode": modes, "loss_grad_ms": ms_lg,
+                            "gd_ms_per_step": ms_gd / 20, "gd_loss_first_last": [float(hist[0]), float(hist[-1])]}
+    del Xd, Fd, Fg
+
+    # sparse: 10000x8000x6000, 20M unique Zipf(1.2) nonzeros, rank 32
+    t0 = time.perf_counter()
+    idx, vals = synth.sparse_zipf()
+    t_gen = time.perf_counter() - t0
+    shape, R, nnz = (10000, 8000, 6000), 32, vals.size
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sp = SparseTensor(idx, vals, shape)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    rng = np.random.default_rng(3)
+    F = [rng.standard_normal((n, R)).astype(np.float32) for n in shape]
+    Fd = [torch.from_numpy(f).to(dev) for f in F]
+    sp.ctx.set_timing(True)
+    modes = {}
+    for mode in range(3):
+        sp.mttkrp(Fd, mode)
+        ts = []
+        for _ in range(5):
+            sp.mttkrp(Fd, mode)
+            ts.append(sp.ctx.last_kernel_ms())
+        t = float(np.median(ts))
+        alg = nnz * (3 * 4 + 4) + sum(shape[k] * R * 4 for k in range(3) if k != mode) + shape[mode] * R * 4
+        modes[str(mode)] = {"ms": t, "GBs": alg / (t * 1e-3) / 1e9, "algorithmic_bytes": alg}
+    sp.ctx.set_timing(False)
+    init = [rng.standard_normal((n, R)).astype(np.float32) for n in shape]
+    sp.cp_als_with_init(init, 2, 0.0)
+    t0 = time.perf_counter()
+    cps = sp.cp_als_with_init(init, 10, 0.0)
+    t_als = time.perf_counter() - t0
+    c5 = {"shape": list(shape), "nnz": int(nnz), "rank": R, "generate_s_host": t_gen, "build_s": t_build,
+          "mttkrp_per_mode": modes, "cp_als_s_per_iter_wall": t_als / 10, "cp_als_fit": cps.fit}
+    if with_cpu:
+        import oracle
+        F64 = [f.astype(np.float64) for f in F]
+        t0 = time.perf_counter()
+        oracle.coo_mttkrp(idx, vals, shape, F64, 0)
+        c5["cpu_coo_mttkrp_mode0_s"] = time.perf_counter() - t0
+        c5["cpu_threads"] = oracle.num_threads()
+    out["sparse_zipf_10000x8000x6000_20M_r32"] = c5
+    sp.close()
+    return out
+
+
 def pmc_probe(Xd, Fd, shape, R):
     """Kernels for the HBM-traffic counter runs: a plain 256 MiB copy (known bytes,
     calibrates FETCH_SIZE / WRITE_SIZE), then 3 launches of each MTTKRP kernel."""
@@ -119,6 +246,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--reps", type=int, default=20, help="launches per MTTKRP kernel timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other workloads (C1, C3, C4, sparse)")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="no timing: launch a 256 MiB device copy (counter calibration) and each MTTKRP kernel a few "
                          "times, for rocprofv3 --pmc runs")
@@ -191,6 +319,8 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(X, init, gpu_fits, K)
 
+    extra = None if args.no_extra else extra_configs(not args.no_cpu_baseline)
+
     out = {
         "metric": METRIC,
         "value": ms32 / 1e3,
@@ -214,6 +344,7 @@ def main():
         "fp32": {"s_per_iter": ms32 / 1e3, "fit": cp32.fit, "wall_s": wall32},
         "mttkrp_per_mode": modes,
         "speedup_vs_cpu_port": (cpu["value"] / (ms32 / 1e3)) if cpu else None,
+        "other_workloads": extra,
         "gpu": torch.cuda.get_device_name(0),
     }
     print(json.dumps(out), flush=True)

@@ -152,6 +152,12 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
     TENRSO_HIP_TRY(hipEventRecord(ctx->ev_fit[k & 1], ctx->stream));
     return TENRSO_OK;
   };
+  // rows of mode n, then their Gram
+  auto solve_and_gram = [&](int n, const MttkrpResult& mr, float* An) -> int {
+    TENRSO_TRY(solve_rows(ctx, mr.partial, mr.splits, nullptr, shape[n], R, Winv, epi, An, ip_part, &nblk));
+    TENRSO_TRY(gram_f64(ctx, An, shape[n], R, G + (i64)n * R * R));
+    return TENRSO_OK;
+  };
   // queue sweep k; with fit_prev its first job also evaluates the fit of sweep k - 1
   auto enqueue_sweep = [&](int k, bool fit_prev) -> int {
     float* const* olds = set[k & 1];
@@ -167,8 +173,7 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
       MttkrpResult mr;
       TENRSO_TRY(pb.mttkrp(n, Fc, &job, &mr));
       if (n == 0 && fit_prev) TENRSO_TRY(publish(k - 1));
-      TENRSO_TRY(solve_rows(ctx, mr.partial, mr.splits, nullptr, shape[n], R, Winv, epi, news[n], ip_part, &nblk));
-      TENRSO_TRY(gram_f64(ctx, news[n], shape[n], R, G + (i64)n * R * R));
+      TENRSO_TRY(solve_and_gram(n, mr, news[n]));
     }
     return TENRSO_OK;
   };

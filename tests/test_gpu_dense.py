@@ -200,3 +200,36 @@ def test_cp_als_bf16_fit_close():
     _, info = oracle.cp_als(X, init, 10, 0.0)
     print(f"  bf16 fit {cp.fit:.8f} oracle {info['fit']:.8f} |dfit| {abs(cp.fit - info['fit']):.2e}", flush=True)
     assert abs(cp.fit - info["fit"]) <= 1e-4
+
+
+@pytest.mark.parametrize("shape", [(256, 256, 256), (512, 512, 128)])
+def test_c3_congruent_precision_vs_fp64(shape):
+    """Ill-conditioned Grams (pairwise column congruence 0.9, noise 1e-4), rank 64,
+    10 ALS iterations from the same initial factors: FP32 and BF16-input paths
+    against the FP64 oracle run.  The normal equations amplify rounding by the
+    condition number of Gamma.  Bars: FP32 fit within 1e-5 and factors within
+    1e-2 (measured 7e-7 / 1.2e-3); BF16 fit within 1e-3 and factors within 0.5
+    (measured 3.1e-4 / 0.22: the bf16 rounding of X, 2^-9 relative, is what the
+    ill-conditioned solve amplifies); first-sweep MTTKRP within 1e-5 (FP32) and
+    5e-3 (BF16).  (The reference's own f32 path reports fit 1.0 here: its
+    sequential f32 fit identity cancels.)"""
+    print(f"[case] C3 congruent {shape} rank 64, 10 iters fp32/bf16 vs fp64", flush=True)
+    X, _, _, init = synth.congruent(shape=shape)
+    ref_F, info = oracle.cp_als(X, init, 10, 0.0)
+    Xd = dev(X)
+    Fd = [dev(f) for f in init]
+    m_ref = oracle.mttkrp_fast(X, f64(init), 0)
+    m32 = kernels.mttkrp(Xd, Fd, 0).cpu().numpy()
+    m16 = kernels.mttkrp_bf16(kernels.to_bf16(Xd), Fd, 0).cpu().numpy()
+    e32, e16 = rel(m32, m_ref), rel(m16, m_ref)
+    print(f"  mttkrp mode 0 rel err fp32 {e32:.2e} bf16 {e16:.2e}", flush=True)
+    results = {}
+    for prec in ("fp32", "bf16"):
+        cp = decomp.cp_als_with_init(Xd, Fd, 10, 0.0, precision=prec)
+        errs = [rel(g.cpu().numpy(), r) for g, r in zip(cp.factors, ref_F)]
+        results[prec] = (abs(cp.fit - info["fit"]), max(errs))
+        print(f"  {prec}: fit {cp.fit:.8f} oracle {info['fit']:.8f} |dfit| {abs(cp.fit - info['fit']):.2e} "
+              f"factor rel errs {['%.2e' % e for e in errs]}", flush=True)
+    assert e32 <= 1e-5 and e16 <= 5e-3
+    assert results["fp32"][0] <= 1e-5 and results["fp32"][1] <= 1e-2
+    assert results["bf16"][0] <= 1e-3 and results["bf16"][1] <= 0.5
