@@ -31,6 +31,8 @@
 
 namespace tenrso {
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
 static inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); }
 
 // ---------------------------------------------------------------- build
@@ -301,6 +303,7 @@ __global__ __launch_bounds__(256) void sp_mttkrp_kernel(SpArgs a, float* __restr
     }
 #pragma unroll
     for (int u = 0; u < SP_U; ++u) {
+      if (e + u >= e1) break;  // past the chunk's end: must not switch rows
       if (rr[u] != cur) {
         // flush row `cur`: the chunk's first row goes to carry 0, later rows are
         // interior (the row changed again inside this chunk) and stored directly
@@ -326,6 +329,139 @@ __global__ __launch_bounds__(256) void sp_mttkrp_kernel(SpArgs a, float* __restr
     c1[col] = acc;
   }
   if (lane == 0) {
+    crow[2 * w] = first;
+    crow[2 * w + 1] = cur;
+  }
+}
+
+// Vector form for R % 4 == 0: a worker is G lanes (lane q owns rank columns
+// r0 + 4q .. 4q+3 and gathers factor rows as 16-byte vectors), a wave holds
+// 64 / G workers, and each worker runs SP_W consecutive sorted nonzeros.  The
+// worker's row / value / index runs are staged SP_SUB at a time into
+// wave-private LDS (dynamic, (nd_oth + 2) arrays of SP_SUB words per worker)
+// with vector loads; the nonzeros are then processed SP_VU at a time with
+// every factor gather issued before any is used.  Same carry contract as
+// sp_mttkrp_kernel: the worker's first row goes to carry slot 2w, its last row
+// to slot 2w + 1, rows strictly inside the run are stored to `out` directly.
+constexpr int SP_W = 512;    // nonzeros per vector worker
+constexpr int SP_SUB = 32;   // nonzeros staged per pass
+constexpr int SP_VU = 8;     // nonzeros in flight per worker
+
+typedef int32_t intx4 __attribute__((ext_vector_type(4)));
+typedef int32_t intx2 __attribute__((ext_vector_type(2)));
+
+template <int G>
+static constexpr size_t sp_vec_lds_bytes(int nd_oth) {
+  return (size_t)4 * (64 / G) * (nd_oth + 2) * SP_SUB * sizeof(int32_t);
+}
+
+// Copy entries [j0, j0 + PER) of a staged run: vector loads when the run is
+// full and 16-byte aligned (an index array of a mode > 0 starts at m * nnz
+// words), element-wise (zero past n) otherwise.
+template <int PER>
+__device__ __forceinline__ void sp_stage_part(const int32_t* __restrict__ src, int32_t* dst, int j0, int n) {
+  if (n == SP_SUB && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+    if constexpr (PER == 2) {
+      *reinterpret_cast<intx2*>(dst + j0) = *reinterpret_cast<const intx2*>(src + j0);
+    } else {
+#pragma unroll
+      for (int k = 0; k < PER; k += 4)
+        *reinterpret_cast<intx4*>(dst + j0 + k) = *reinterpret_cast<const intx4*>(src + j0 + k);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) dst[j0 + k] = (j0 + k < n) ? src[j0 + k] : 0;
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(256) void sp_mttkrp_vec_kernel(SpArgs a, float* __restrict__ out,
+                                                            float* __restrict__ cval, int32_t* __restrict__ crow) {
+  constexpr int NPW = 64 / G;      // workers per wave
+  constexpr int PER = SP_SUB / G;  // staged entries per lane and array
+  static_assert(PER == 2 || PER % 4 == 0, "staging width");
+  extern __shared__ int32_t sp_stage[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane / G, q = lane % G;
+  const int na = a.nd_oth + 2;  // row, value, the other modes' indices
+  int32_t* st = sp_stage + ((wave * NPW + g) * na) * SP_SUB;
+  const i64 w = ((i64)blockIdx.x * 4 + wave) * NPW + g;
+  const bool active = w < a.nworkers;
+  const i64 e0 = w * SP_W;
+  const i64 e1 = active ? ((e0 + SP_W) < a.nnz ? e0 + SP_W : a.nnz) : e0;
+  const bool live = 4 * q < a.rcount;
+  const int col = a.r0 + (live ? 4 * q : 0);
+  const int first = active ? a.row[e0] : 0;
+  int cur = first;
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  auto flush = [&](int row, const floatx4& v) {
+    if (!live) return;
+    float* dst = (row == first) ? cval + (2 * w) * a.R + col : out + (i64)row * a.R + col;
+    *reinterpret_cast<floatx4*>(dst) = v;
+  };
+  for (i64 base = e0; base < e1; base += SP_SUB) {
+    const int n = (int)((e1 - base) < SP_SUB ? (e1 - base) : SP_SUB);
+    sp_stage_part<PER>(a.row + base, st, q * PER, n);
+    sp_stage_part<PER>(reinterpret_cast<const int32_t*>(a.vals) + base, st + SP_SUB, q * PER, n);
+    for (int m = 0; m < a.nd_oth; ++m) sp_stage_part<PER>(a.oth + (i64)m * a.nnz + base, st + (2 + m) * SP_SUB, q * PER, n);
+    // LDS traffic of one wave is processed in order: the group's reads below
+    // see its writes above; the barrier only keeps the compiler from moving them
+    __builtin_amdgcn_wave_barrier();
+    for (int i0 = 0; i0 < n; i0 += SP_VU) {
+      int rr[SP_VU];
+      floatx4 pv[SP_VU];
+#pragma unroll
+      for (int u = 0; u < SP_VU; u += 4) {
+        const intx4 r4 = *reinterpret_cast<const intx4*>(st + i0 + u);
+        const intx4 v4 = *reinterpret_cast<const intx4*>(st + SP_SUB + i0 + u);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          rr[u + k] = r4[k];
+          const float v = __int_as_float(v4[k]);
+          pv[u + k] = floatx4{v, v, v, v};
+        }
+      }
+      for (int m = 0; m < a.nd_oth; ++m) {
+        const float* Fm = a.F[m] + col;
+        floatx4 fv[SP_VU];
+#pragma unroll
+        for (int u = 0; u < SP_VU; u += 4) {
+          const intx4 x4 = *reinterpret_cast<const intx4*>(st + (2 + m) * SP_SUB + i0 + u);
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            // staged entries past n are zero (row 0 is a valid factor row)
+            fv[u + k] = *reinterpret_cast<const floatx4*>(Fm + (i64)x4[k] * a.R);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < SP_VU; ++u) pv[u] *= fv[u];
+      }
+#pragma unroll
+      for (int u = 0; u < SP_VU; ++u) {
+        if (i0 + u >= n) break;  // past the run's end: must not switch rows
+        if (rr[u] != cur) {
+          flush(cur, acc);
+          cur = rr[u];
+          acc = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+        acc += pv[u];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // staged entries read before the next pass overwrites them
+  }
+  if (!active) return;
+  // the worker's last row is a carry; a single-row worker emits (first, acc) and (first, 0)
+  if (live) {
+    float* c0 = cval + (2 * w) * a.R + col;
+    float* c1 = cval + (2 * w + 1) * a.R + col;
+    if (cur == first) {
+      *reinterpret_cast<floatx4*>(c0) = acc;
+      *reinterpret_cast<floatx4*>(c1) = floatx4{0.f, 0.f, 0.f, 0.f};
+    } else {
+      *reinterpret_cast<floatx4*>(c1) = acc;
+    }
+  }
+  if (q == 0) {
     crow[2 * w] = first;
     crow[2 * w + 1] = cur;
   }
@@ -393,10 +529,17 @@ __global__ __launch_bounds__(256) void zero_f32_kernel(float* __restrict__ p, i6
 
 template <int LPN>
 static int sp_mttkrp_launch(tenrso_ctx* ctx, SpArgs& a, float* out, float* cv0, int32_t* ck0, float* cv1,
-                            int32_t* ck1) {
+                            int32_t* ck1, bool vec) {
   const i64 R = a.R;
-  hipLaunchKernelGGL(sp_mttkrp_kernel<LPN>, dim3(cdiv(a.nworkers * LPN, 256)), dim3(256), 0, ctx->stream, a, out,
-                     cv0, ck0);
+  if (vec) {
+    constexpr int G = LPN / 4;
+    constexpr int NPW = 64 / G;
+    hipLaunchKernelGGL(sp_mttkrp_vec_kernel<G>, dim3(cdiv(a.nworkers, 4 * NPW)), dim3(256),
+                       sp_vec_lds_bytes<G>(a.nd_oth), ctx->stream, a, out, cv0, ck0);
+  } else {
+    hipLaunchKernelGGL(sp_mttkrp_kernel<LPN>, dim3(cdiv(a.nworkers * LPN, 256)), dim3(256), 0, ctx->stream, a, out,
+                       cv0, ck0);
+  }
   i64 S = 2 * a.nworkers;
   float* iv = cv0;
   int32_t* ik = ck0;
@@ -434,7 +577,11 @@ int sp_mttkrp(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* F, 
   int o = 0;
   for (int k = 0; k < t->nd; ++k)
     if (k != mode) a.F[o++] = F[k];
-  a.nworkers = (t->nnz + SP_CH - 1) / SP_CH;
+  // 16-byte factor gathers and stores need rank columns in groups of 4 and
+  // 16-byte aligned factor / output bases
+  bool vec = (R % 4) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+  for (int k = 0; k < a.nd_oth; ++k) vec = vec && (reinterpret_cast<uintptr_t>(a.F[k]) & 15) == 0;
+  a.nworkers = vec ? (t->nnz + SP_W - 1) / SP_W : (t->nnz + SP_CH - 1) / SP_CH;
   const i64 nslots = 2 * a.nworkers;
   void *pv, *pk;
   TENRSO_TRY(ctx_buffer(ctx, SLOT_TMP, (size_t)2 * nslots * R * sizeof(float), &pv));
@@ -448,9 +595,9 @@ int sp_mttkrp(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* F, 
   for (int rc = 0; rc < rchunks; ++rc) {
     a.r0 = rc * LPN;
     a.rcount = (int)std::min<i64>(LPN, R - a.r0);
-    if (LPN == 16) TENRSO_TRY(sp_mttkrp_launch<16>(ctx, a, out, cv0, ck0, cv1, ck1));
-    else if (LPN == 32) TENRSO_TRY(sp_mttkrp_launch<32>(ctx, a, out, cv0, ck0, cv1, ck1));
-    else TENRSO_TRY(sp_mttkrp_launch<64>(ctx, a, out, cv0, ck0, cv1, ck1));
+    if (LPN == 16) TENRSO_TRY(sp_mttkrp_launch<16>(ctx, a, out, cv0, ck0, cv1, ck1, vec));
+    else if (LPN == 32) TENRSO_TRY(sp_mttkrp_launch<32>(ctx, a, out, cv0, ck0, cv1, ck1, vec));
+    else TENRSO_TRY(sp_mttkrp_launch<64>(ctx, a, out, cv0, ck0, cv1, ck1, vec));
   }
   return TENRSO_OK;
 }

@@ -129,6 +129,36 @@ def test_sparse_skewed_rows_and_edge_cases():
         SparseTensor(np.array([[0], [9], [0]], np.int32), np.array([1.0], np.float32), (2, 3, 4))
 
 
+@pytest.mark.parametrize("R", [3, 4, 8, 68])
+def test_sparse_run_tail_row_changes(R):
+    # 1027 nonzeros: the last worker run (1024 nonzeros per scalar chunk, 512 per
+    # vector run) holds 3 nonzeros in 3 different rows, so its final partial batch
+    # changes rows twice and then runs past the end
+    print(f"[case] sparse run tail R={R}", flush=True)
+    rng = np.random.default_rng(R)
+    shape = (20, 30, 40)
+    head = rng.choice(17 * 30 * 40, size=1024, replace=False)
+    tail = np.array([np.ravel_multi_index((r, 1, 2), shape) for r in (17, 18, 19)])
+    lin = np.concatenate([head, tail])
+    idx = np.array(np.unravel_index(lin, shape), dtype=np.int32)
+    vals = rng.random(lin.size).astype(np.float32)
+    sp = SparseTensor(idx, vals, shape)
+    F, Fd = dev_factors(shape, R, seed=R + 1)
+    F64 = [f.astype(np.float64) for f in F]
+    for mode in range(3):
+        got = sp.mttkrp(Fd, mode).cpu().numpy()
+        assert rel(got, oracle.coo_mttkrp(idx, vals, shape, F64, mode)) <= 1e-5
+    # factors that start 4 bytes into their allocation take the scalar path
+    Fo = []
+    for f in F:
+        buf = torch.empty(f.size + 1, dtype=torch.float32, device="cuda")
+        buf[1:] = torch.from_numpy(f.reshape(-1)).cuda()
+        Fo.append(buf[1:].view(f.shape))
+    assert Fo[0].data_ptr() % 16 == 4
+    got = sp.mttkrp(Fo, 0).cpu().numpy()
+    assert rel(got, oracle.coo_mttkrp(idx, vals, shape, F64, 0)) <= 1e-5
+
+
 def test_sparse_mttkrp_deterministic():
     idx, vals = synth.sparse_zipf(shape=(1000, 800, 600), nnz=200_000, seed=1)
     sp = SparseTensor(idx, vals, (1000, 800, 600))
