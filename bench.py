@@ -250,6 +250,8 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="no timing: launch a 256 MiB device copy (counter calibration) and each MTTKRP kernel a few "
                          "times, for rocprofv3 --pmc runs")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the per-mode MTTKRP kernel timings (--reps launches each), for rocprofv3 --stats runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding across GPUs is not part of "
@@ -283,23 +285,30 @@ def main():
         wall = time.perf_counter() - t0
         return cp, e0.elapsed_time(e1) / K, wall
 
+    def kernel_timings():
+        # per-mode MTTKRP kernels, timed with HIP events on the launch stream
+        ctx = _lib.torch_context(Xd)
+        Xb = kernels.to_bf16(Xd)
+        torch.cuda.synchronize()
+        modes = {}
+        for mode in range(len(shape)):
+            t32 = profile_kernel(ctx, _lib.PREC_FP32, Xd, shape, Fd, R, mode, args.reps)
+            t16 = profile_kernel(ctx, _lib.PREC_BF16, Xb, shape, Fd, R, mode, args.reps)
+            b32, b16 = mttkrp_bytes(shape, R, mode, 4), mttkrp_bytes(shape, R, mode, 2)
+            fl = mttkrp_flops(shape, R)
+            modes[str(mode)] = {
+                "fp32_us": t32 * 1e3, "fp32_GBs": b32 / (t32 * 1e-3) / 1e9, "fp32_TFLOPs": fl / (t32 * 1e-3) / 1e12,
+                "bf16_us": t16 * 1e3, "bf16_GBs": b16 / (t16 * 1e-3) / 1e9, "bf16_TFLOPs": fl / (t16 * 1e-3) / 1e12,
+            }
+        return modes
+
+    if args.kernels_only:
+        print(json.dumps({"mttkrp_per_mode": kernel_timings(), "reps": args.reps}), flush=True)
+        return
+
     cp32, ms32, wall32 = timed_als("fp32")
     cp16, ms16, wall16 = timed_als("bf16")
-
-    # per-mode MTTKRP kernels, timed with HIP events on the launch stream
-    ctx = _lib.torch_context(Xd)
-    Xb = kernels.to_bf16(Xd)
-    torch.cuda.synchronize()
-    modes = {}
-    for mode in range(len(shape)):
-        t32 = profile_kernel(ctx, _lib.PREC_FP32, Xd, shape, Fd, R, mode, args.reps)
-        t16 = profile_kernel(ctx, _lib.PREC_BF16, Xb, shape, Fd, R, mode, args.reps)
-        b32, b16 = mttkrp_bytes(shape, R, mode, 4), mttkrp_bytes(shape, R, mode, 2)
-        fl = mttkrp_flops(shape, R)
-        modes[str(mode)] = {
-            "fp32_us": t32 * 1e3, "fp32_GBs": b32 / (t32 * 1e-3) / 1e9, "fp32_TFLOPs": fl / (t32 * 1e-3) / 1e12,
-            "bf16_us": t16 * 1e3, "bf16_GBs": b16 / (t16 * 1e-3) / 1e9, "bf16_TFLOPs": fl / (t16 * 1e-3) / 1e12,
-        }
+    modes = kernel_timings()
     # dominant kernel: the fp32 MFMA instance of modes 0 and 1 (q-contiguous tiles)
     t_dom = (modes["0"]["fp32_us"] + modes["1"]["fp32_us"]) / 2.0
     ach = mttkrp_flops(shape, R) / (t_dom * 1e-6) / 1e12

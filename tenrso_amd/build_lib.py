@@ -4,7 +4,8 @@ Every .hip source under tenrso_amd/csrc is compiled with hipcc for gfx950 only,
 in parallel, then linked into one shared library exporting the C ABI of
 include/tenrso_hip.h.  Packed FP32 instructions are disabled
 (-target-feature -packed-fp32-ops); hipcc's host pass warns that it does not
-know that device feature, which is expected.
+know that device feature, which is expected.  MFMA operands are kept in VGPRs
+(-amdgpu-mfma-vgpr-form).
 """
 import concurrent.futures as cf
 import glob
@@ -20,6 +21,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # MFMA accumulators in ordinary VGPRs: the FP64 job edits its
+         # accumulators with VALU ops between MFMAs, which in AGPRs costs a
+         # copy out and back per step
+         "-mllvm", "-amdgpu-mfma-vgpr-form",
          "-I", os.path.join(os.path.dirname(HERE), "include")]
 
 

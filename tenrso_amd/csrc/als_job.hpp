@@ -25,12 +25,10 @@ namespace tenrso {
 
 constexpr int JOB_NE = 64;        // padded matrix order (rank bound)
 constexpr int JOB_THREADS = 256;  // workgroup size of the job
-// GJ: [2][16] pivot blocks, [2][4][64] pivot rows, [2][64][4] pivot columns,
-// [64][8] regularisation band.  Thread t owns the 4 x 4 block (t / 16, t % 16)
-// of every 64 x 64 matrix.
-constexpr int JOB_GJ_P = 0, JOB_GJ_ROW = 32, JOB_GJ_Q = JOB_GJ_ROW + 2 * 4 * JOB_NE;
-constexpr int JOB_GJ_REG = JOB_GJ_Q + 2 * 4 * JOB_NE;
-constexpr int JOB_LDS_BYTES = (JOB_GJ_REG + JOB_NE * 8) * 8;  // LDS bytes of the job
+// GJ (doubles): [2][4][64] pivot rows, [2][4][4] pivot-block inverses, then
+// one int flag.
+constexpr int JOB_GJ_ROW = 0, JOB_GJ_PINV = 2 * 4 * JOB_NE, JOB_GJ_FLAG = JOB_GJ_PINV + 2 * 16;
+constexpr int JOB_LDS_BYTES = (JOB_GJ_FLAG + 2) * 8;  // LDS bytes of the job
 
 // Device scalar block slots shared with the ALS driver.
 constexpr int JOB_SC_NORM_X = 0, JOB_SC_FIT = 1, JOB_SC_IP = 2, JOB_SC_RECON = 3, JOB_SC_STATUS = 4;
@@ -56,17 +54,6 @@ __device__ __forceinline__ double job_recip(double p) {
   r = fma(r, fma(-p, r, 1.0), r);
   r = fma(r, fma(-p, r, 1.0), r);
   return r;
-}
-
-// 1/sqrt(d): hardware estimate refined by Newton steps.
-#ifndef TENRSO_JOB_RSQ_STEPS
-#define TENRSO_JOB_RSQ_STEPS 2
-#endif
-__device__ __forceinline__ double job_rsqrt(double d) {
-  double y = __builtin_amdgcn_rsq(d);
-#pragma unroll
-  for (int i = 0; i < TENRSO_JOB_RSQ_STEPS; ++i) y = y * fma(-0.5 * d * y, y, 1.5);
-  return y;
 }
 
 __device__ __forceinline__ double job_block_sum(double v, double* red) {
@@ -110,117 +97,174 @@ __device__ __forceinline__ double job_reg(const AlsJob& a, int i, int j, double 
 
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-// Block Gauss-Jordan inverse of the 64 x 64 matrix held as 4 x 4 blocks:
-// thread t owns av[u][v] = A[4bi + u][4bj + v], bi = t / 16, bj = t % 16.
-// Step s eliminates the pivot block J = {4s..4s+3}, P = A[J][J]:
-//   W = P^-1 Arow, Arow = A[J][:] with columns J replaced by the identity
-//       (each thread solves P w = Arow[:, c] for its four columns with a 4 x 4
-//       Cholesky factorisation of P);
-//   pivot rows:  A[J][:] <- W;
-//   other rows:  A[i][c] <- Ahat[i][c] - sum_t A[i][J_t] W[t][c],
-//                Ahat = A with columns J replaced by zeros.
-// This leaves A - A[:,J] P^-1 A[J,:] off the block row and column, P^-1 A[J,:]
-// on the pivot rows, -A[:,J] P^-1 on the pivot columns and P^-1 on the block.
-// The threads of block row s publish the pivot rows, those of block column s
-// the pivot columns.  Each thread reads 16 pivot-row and 16 pivot-column values
-// per step; LDS is double-buffered, so one barrier per step suffices.  Returns
-// false if a Cholesky pivot is not positive and finite (the matrix is not
-// positive definite).
-__device__ __forceinline__ bool job_block_gj(double (&av)[4][4], double* lds) {
-  const int bi = threadIdx.x >> 4, bj = threadIdx.x & 15;
-  bool ok = true;
-#pragma unroll 1
-  for (int s = 0; s < 16; ++s) {
-    const int b = s & 1;
-    double* P = lds + JOB_GJ_P + b * 16;
-    double* Rw = lds + JOB_GJ_ROW + b * 4 * JOB_NE;
-    double* Q = lds + JOB_GJ_Q + b * 4 * JOB_NE;
-    const bool prow = bi == s, pcol = bj == s;
-    if (prow) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        double2 lo = make_double2(av[u][0], av[u][1]), hi = make_double2(av[u][2], av[u][3]);
-        if (pcol) {
-          lo = make_double2(u == 0 ? 1.0 : 0.0, u == 1 ? 1.0 : 0.0);
-          hi = make_double2(u == 2 ? 1.0 : 0.0, u == 3 ? 1.0 : 0.0);
-        }
-        *reinterpret_cast<double2*>(&Rw[u * JOB_NE + 4 * bj]) = lo;
-        *reinterpret_cast<double2*>(&Rw[u * JOB_NE + 4 * bj + 2]) = hi;
-      }
-      if (pcol) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) P[u * 4 + v] = av[u][v];
-      }
-    }
-    if (pcol) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        *reinterpret_cast<double2*>(&Q[(4 * bi + u) * 4]) = make_double2(av[u][0], av[u][1]);
-        *reinterpret_cast<double2*>(&Q[(4 * bi + u) * 4 + 2]) = make_double2(av[u][2], av[u][3]);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) av[u][v] = 0.0;
-      }
-    }
-    __syncthreads();
-    TENRSO_JOB_MARK(10 + s);
-    // all LDS reads of the step are issued before the Cholesky chain
-    const double p00 = P[0], p10 = P[4], p11 = P[5], p20 = P[8], p21 = P[9], p22 = P[10];
-    const double p30 = P[12], p31 = P[13], p32 = P[14], p33 = P[15];
-    double r[4][4], q[4][4];
+// Inverse of a symmetric 2 x 2 matrix [[a, b], [b, d]]; ok stays true only
+// if it is positive definite.
+__device__ __forceinline__ void job_inv2(double a, double b, double d, double& ia, double& ib, double& id, bool& ok) {
+  const double det = fma(a, d, -(b * b));
+  ok = ok && a > 0.0 && det > 0.0 && det <= DBL_MAX;  // NaN fails every comparison
+  const double r = job_recip(det);
+  ia = d * r;
+  ib = -b * r;
+  id = a * r;
+}
+
+// Inverse of a symmetric positive definite 4 x 4 block P = [[A, B], [B^T, D]]
+// (2 x 2 blocks) through the Schur complement S = D - B^T A^-1 B:
+//   P^-1 = [[A^-1 + Y X^T, -Y], [-Y^T, S^-1]],  X = A^-1 B,  Y = X S^-1.
+// ok becomes false if A or S is not positive definite (or not finite).
+__device__ __forceinline__ void job_inv4_spd(const double (&p)[4][4], double (&o)[4][4], bool& ok) {
+  double a00, a01, a11;
+  job_inv2(p[0][0], p[0][1], p[1][1], a00, a01, a11, ok);
+  const double x00 = fma(a00, p[0][2], a01 * p[1][2]), x01 = fma(a00, p[0][3], a01 * p[1][3]);
+  const double x10 = fma(a01, p[0][2], a11 * p[1][2]), x11 = fma(a01, p[0][3], a11 * p[1][3]);
+  const double s00 = p[2][2] - fma(p[0][2], x00, p[1][2] * x10);
+  const double s01 = p[2][3] - fma(p[0][2], x01, p[1][2] * x11);
+  const double s11 = p[3][3] - fma(p[0][3], x01, p[1][3] * x11);
+  double i00, i01, i11;
+  job_inv2(s00, s01, s11, i00, i01, i11, ok);
+  const double y00 = fma(x00, i00, x01 * i01), y01 = fma(x00, i01, x01 * i11);
+  const double y10 = fma(x10, i00, x11 * i01), y11 = fma(x10, i01, x11 * i11);
+  o[0][0] = fma(y00, x00, fma(y01, x01, a00));
+  o[0][1] = fma(y00, x10, fma(y01, x11, a01));
+  o[1][1] = fma(y10, x10, fma(y11, x11, a11));
+  o[1][0] = o[0][1];
+  o[0][2] = -y00;
+  o[0][3] = -y01;
+  o[1][2] = -y10;
+  o[1][3] = -y11;
+  o[2][0] = -y00;
+  o[3][0] = -y01;
+  o[2][1] = -y10;
+  o[3][1] = -y11;
+  o[2][2] = i00;
+  o[2][3] = i01;
+  o[3][2] = i01;
+  o[3][3] = i11;
+}
+
+// Matrix layout of the job: wave w (of 4) holds rows 16w..16w+15 of the
+// 64 x 64 matrix as four 16 x 16 tiles in the accumulator layout of
+// v_mfma_f64_16x16x4f64: lane l, tile t, element e holds
+//   A[16w + 4e + l/16][16t + l%16].
+// In that layout rows 4m..4m+3 of a tile (element e = m) are exactly a 4 x 16
+// B operand of the same instruction (lane l holds B[l/16][l%16]).
+__device__ __forceinline__ int job_row(int e) { return 16 * (threadIdx.x >> 6) + 4 * e + ((threadIdx.x & 63) >> 4); }
+__device__ __forceinline__ int job_col(int t) { return 16 * t + (threadIdx.x & 15); }
+
+// Block Gauss-Jordan inverse (4 x 4 pivot blocks) of a symmetric positive
+// definite 64 x 64 matrix in the job layout, on the FP64 matrix cores.
+// Step s eliminates J = {4s..4s+3}, P = A[J][J]:
+//   W = P^-1 Arow', Arow' = A[J][:] with columns J replaced by the identity;
+//   A <- Ahat - A[:,J] W, Ahat = A with rows J and columns J zeroed, and
+//   with A[J][J] taken as -I in that product,
+// which leaves W on the pivot rows, -A[:,J] P^-1 on the pivot columns, P^-1
+// on the block and A - A[:,J] P^-1 A[J,:] elsewhere.  The matrix stays
+// symmetric up to sign (A[i][J] = A[J][i] for rows i not yet eliminated,
+// -A[J][i] for eliminated ones), so the pivot column operand is read from the
+// published pivot rows.
+// Per step, after the barrier, every wave forms W for all 64 columns with four
+// v_mfma_f64_4x4x4f64 (block b of tile t = columns 16t + 4b..+3; lane l holds
+// A[l&3][l>>4], B[l>>4][l&3], D[l>>4][l&3], so D comes out in the B operand
+// layout of the 16x16x4 instruction) and updates its four tiles with four
+// v_mfma_f64_16x16x4f64, the tile holding the next pivot block first.  The
+// wave holding the next pivot rows publishes them (one 8-byte store per lane
+// and tile) and one of its lanes inverts the next P (job_inv4_spd) while the
+// remaining MFMAs run.  LDS is double-buffered: one barrier per step.  Returns
+// false if a pivot block is not positive definite.
+__device__ __forceinline__ bool job_block_gj(doublex4 (&acc)[4], double* lds) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int q = lane >> 4, j = lane & 15;
+  int* flag = reinterpret_cast<int*>(lds + JOB_GJ_FLAG);
+  if (threadIdx.x == 0) *flag = 0;
+  // lane 0 of the wave holding rows 4s..4s+3: P from the published rows -> P^-1
+  auto invert_pivot = [&](int s) {
+    const double* Rw = lds + JOB_GJ_ROW + (s & 1) * 4 * JOB_NE;
+    double* Pi = lds + JOB_GJ_PINV + (s & 1) * 16;
+    double p[4][4], pinv[4][4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const double2 ra = *reinterpret_cast<const double2*>(&Rw[u * JOB_NE + 4 * bj]);
-      const double2 rb = *reinterpret_cast<const double2*>(&Rw[u * JOB_NE + 4 * bj + 2]);
-      r[u][0] = ra.x;
-      r[u][1] = ra.y;
-      r[u][2] = rb.x;
-      r[u][3] = rb.y;
-      const double2 qa = *reinterpret_cast<const double2*>(&Q[(4 * bi + u) * 4]);
-      const double2 qb = *reinterpret_cast<const double2*>(&Q[(4 * bi + u) * 4 + 2]);
-      q[u][0] = qa.x;
-      q[u][1] = qa.y;
-      q[u][2] = qb.x;
-      q[u][3] = qb.y;
+      const double2 lo = *reinterpret_cast<const double2*>(&Rw[u * JOB_NE + 4 * s]);
+      const double2 hi = *reinterpret_cast<const double2*>(&Rw[u * JOB_NE + 4 * s + 2]);
+      p[u][0] = lo.x;
+      p[u][1] = lo.y;
+      p[u][2] = hi.x;
+      p[u][3] = hi.y;
     }
-    // Cholesky P = L L^T (lower triangle of P), redundantly in every thread
-    const double i0 = job_rsqrt(p00);
-    const double l10 = p10 * i0, l20 = p20 * i0, l30 = p30 * i0;
-    const double d1 = fma(-l10, l10, p11);
-    const double i1 = job_rsqrt(d1);
-    const double l21 = fma(-l20, l10, p21) * i1, l31 = fma(-l30, l10, p31) * i1;
-    const double d2 = fma(-l21, l21, fma(-l20, l20, p22));
-    const double i2 = job_rsqrt(d2);
-    const double l32 = fma(-l31, l21, fma(-l30, l20, p32)) * i2;
-    const double d3 = fma(-l32, l32, fma(-l31, l31, fma(-l30, l30, p33)));
-    const double i3 = job_rsqrt(d3);
-    ok = ok && p00 > 0.0 && d1 > 0.0 && d2 > 0.0 && d3 > 0.0 && isfinite(i0 * i1 * i2 * i3);
-    TENRSO_JOB_MARK(30 + (s & 3));
-    // for each of the four columns: L y = Arow[:, c], L^T w = y
-    double wv[4][4];
+    bool ok = true;
+    job_inv4_spd(p, pinv, ok);
+    if (!ok) *flag = 1;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const double y0 = r[0][v] * i0;
-      const double y1 = fma(-l10, y0, r[1][v]) * i1;
-      const double y2 = fma(-l21, y1, fma(-l20, y0, r[2][v])) * i2;
-      const double y3 = fma(-l32, y2, fma(-l31, y1, fma(-l30, y0, r[3][v]))) * i3;
-      wv[3][v] = y3 * i3;
-      wv[2][v] = fma(-l32, wv[3][v], y2) * i2;
-      wv[1][v] = fma(-l31, wv[3][v], fma(-l21, wv[2][v], y1)) * i1;
-      wv[0][v] = fma(-l30, wv[3][v], fma(-l20, wv[2][v], fma(-l10, wv[1][v], y0))) * i0;
+    for (int r = 0; r < 4; ++r) {
+      *reinterpret_cast<double2*>(&Pi[r * 4]) = make_double2(pinv[r][0], pinv[r][1]);
+      *reinterpret_cast<double2*>(&Pi[r * 4 + 2]) = make_double2(pinv[r][2], pinv[r][3]);
     }
+  };
+  // step 0's pivot rows and inverse
+  if (wave == 0) {
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int t = 0; t < 4; ++t) lds[JOB_GJ_ROW + q * JOB_NE + 16 * t + j] = acc[t][0];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    if (lane == 0) invert_pivot(0);
+  }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const double upd =
-            fma(-q[u][3], wv[3][v], fma(-q[u][2], wv[2][v], fma(-q[u][1], wv[1][v], fma(-q[u][0], wv[0][v], av[u][v]))));
-        av[u][v] = prow ? wv[u][v] : upd;
+  for (int s = 0; s < 16; ++s) {
+    const int b = s & 1, T = s >> 2, e = s & 3;
+    const double* Rw = lds + JOB_GJ_ROW + b * 4 * JOB_NE;
+    const double* Pi = lds + JOB_GJ_PINV + b * 16;
+    __syncthreads();
+    TENRSO_JOB_MARK(10 + s);
+    const double pa = Pi[(lane & 3) * 4 + q];  // P^-1[l & 3][l >> 4]
+    double rb[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) rb[t] = Rw[q * JOB_NE + 16 * t + j];  // Arow[q][16t + j]
+    const double rown = Rw[q * JOB_NE + 16 * wave + j];            // Arow[q][16 wave + j]
+    // Arow': identity on the pivot columns
+    const bool pj = (j >> 2) == e;  // column 16T + j is a pivot column of tile T
+    rb[T] = pj ? ((j & 3) == q ? 1.0 : 0.0) : rb[T];
+    // W[q][16t + j] for the four tiles
+    double wv[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) wv[t] = __builtin_amdgcn_mfma_f64_4x4x4f64(pa, rb[t], 0.0, 0, 0, 0);
+    // A operand: -A[16 wave + j][J_q] (rows of this wave), -I -> +I on the pivot rows
+    const int c = 16 * wave + j;
+    double aop;
+    if (wave == T && pj) aop = (j & 3) == q ? 1.0 : 0.0;
+    else aop = c < 4 * s ? rown : -rown;
+    // Ahat: zero the pivot columns (tile T) and, in wave T, the pivot rows
+    if (pj) acc[T] = doublex4{0.0, 0.0, 0.0, 0.0};
+    if (wave == T) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t][e] = 0.0;
+    }
+    if (s + 1 < 16) {
+      const int s1 = s + 1, T1 = s1 >> 2, e1 = s1 & 3;
+      double* Rn = lds + JOB_GJ_ROW + (s1 & 1) * 4 * JOB_NE;
+      // the tile holding the next pivot block first
+      acc[T1] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop, wv[T1], acc[T1], 0, 0, 0);
+#pragma unroll
+      for (int i = 1; i < 4; ++i) {
+        const int t = (T1 + i) & 3;
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop, wv[t], acc[t], 0, 0, 0);
       }
+      if (wave == T1) {
+        Rn[q * JOB_NE + 16 * T1 + j] = acc[T1][e1];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        if (lane == 0) invert_pivot(s1);
+#pragma unroll
+        for (int i = 1; i < 4; ++i) {
+          const int t = (T1 + i) & 3;
+          Rn[q * JOB_NE + 16 * t + j] = acc[t][e1];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop, wv[t], acc[t], 0, 0, 0);
+    }
     TENRSO_JOB_MARK(40 + (s & 3));
   }
   __syncthreads();
+  const bool ok = *flag == 0;
+  __syncthreads();  // every thread has read the flag before a retry resets it
   return ok;
 }
 
@@ -228,42 +272,53 @@ __device__ __forceinline__ bool job_block_gj(double (&av)[4][4], double* lds) {
 // threads of the workgroup must call it.
 __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
   const int R = a.R;
-  // every Gram entry a thread touches is in its 4 x 4 block (t / 16, t % 16)
-  const int bi = threadIdx.x >> 4, bj = threadIdx.x & 15;
+  // entries of this thread: (job_row(e), job_col(t)), t, e = 0..3
+  // Inside an MTTKRP launch the job's waves share their SIMDs with MTTKRP
+  // waves; the job is a chain of short dependent steps, so it takes issue
+  // priority and the MTTKRP waves fill its latency gaps.
+  __builtin_amdgcn_s_setprio(3);
   TENRSO_JOB_MARK(0);
-  // entries of G_k in this thread's block
-  auto gram_block = [&](int k, double (&g)[4][4]) {
-    const double* Gk = a.G + (i64)k * R * R;
+  const bool want_fit = a.do_fit != 0;
+  // <X, Xhat> partials: loads issued ahead of the Gram loads
+  double ipv = 0.0;
+  if (want_fit)
+    for (int b = threadIdx.x; b < a.nblk; b += JOB_THREADS) ipv += a.ip_part[b];
+  // one pass over the Grams: cross = hadamard of all (fit), gam = hadamard of
+  // k != skip (Gamma, ascending k from ones)
+  double cross[4][4], gam[4][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = 4 * bi + u, j = 4 * bj + v;
-        g[u][v] = (i < R && j < R) ? Gk[i * R + j] : 0.0;
-      }
-  };
-  if (a.do_fit) {
-    // ||Xhat||^2 = sum_{r,s} prod_k G_k[r,s]; <X, Xhat> = sum of the partials
-    double cross[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) cross[u][v] = 1.0;
-    for (int k = 0; k < a.nd; ++k) {
-      double g[4][4];
-      gram_block(k, g);
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) cross[u][v] *= g[u][v];
+    for (int e = 0; e < 4; ++e) {
+      cross[t][e] = 1.0;
+      gam[t][e] = 1.0;
     }
+  for (int k = 0; k < a.nd; ++k) {
+    const double* Gk = a.G + (i64)k * R * R;
+    double g[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = job_row(e), j = job_col(t);
+        g[t][e] = (i < R && j < R) ? Gk[i * R + j] : 0.0;
+      }
+    const bool in_gam = k != a.skip;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        cross[t][e] *= g[t][e];
+        gam[t][e] = in_gam ? gam[t][e] * g[t][e] : gam[t][e];
+      }
+  }
+  if (want_fit) {
+    // ||Xhat||^2 = sum_{r,s} prod_k G_k[r,s]; <X, Xhat> = sum of the partials
     double rec = 0.0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) rec += cross[u][v];
-    double ipv = 0.0;
-    for (int b = threadIdx.x; b < a.nblk; b += JOB_THREADS) ipv += a.ip_part[b];
+      for (int e = 0; e < 4; ++e) rec += cross[t][e];
     const double recon = job_block_sum(rec, lds);
     const double ip = job_block_sum(ipv, lds + 8);
     if (threadIdx.x == 0) {
@@ -280,57 +335,36 @@ __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
   }
   TENRSO_JOB_MARK(2);
   if (a.skip < 0) return;
-  // Gamma = hadamard_{k != skip} G_k (ascending k, from ones) + regularisation,
-  // identity outside R x R, in the 4 x 4 block layout of job_block_gj
-  double* band = lds + JOB_GJ_REG;  // [64][8]: regularisation of entries (i, i + d), d = -2..2
+  // Gamma + regularisation, identity outside R x R, in the 4 x 4 block layout
   int path = 0;
   bool ok = false;
-  double av[4][4];
+  // Gamma waits in Winv (global) for the ridge retry instead of in registers
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) a.Winv[job_row(e) * JOB_NE + job_col(t)] = gam[t][e];
+  doublex4 acc[4];
 #pragma unroll 1
   for (int attempt = 0; attempt < 2 && !ok; ++attempt) {
     const double ridge = attempt ? (double)FLT_EPSILON * (double)(10 * R) : 0.0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) av[u][v] = 1.0;
-    for (int k = 0; k < a.nd; ++k) {
-      if (k == a.skip) continue;
-      double g[4][4];
-      gram_block(k, g);
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) av[u][v] *= g[u][v];
-    }
-    __syncthreads();
-    if (threadIdx.x < JOB_NE) {
-      const int i = threadIdx.x;
-#pragma unroll 1
-      for (int d = -2; d <= 2; ++d) {
-        const int col = i + d;
-        band[i * 8 + d + 2] = (i < R && col >= 0 && col < R) ? job_reg(a, i, col, ridge) : 0.0;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = 4 * bi + u, col = 4 * bj + v;
-        const int d = col - i;
-        const double add = (d >= -2 && d <= 2) ? band[i * 8 + d + 2] : 0.0;
-        av[u][v] = (i < R && col < R) ? av[u][v] + add : (i == col ? 1.0 : 0.0);
+      for (int e = 0; e < 4; ++e) {
+        const int i = job_row(e), col = job_col(t);
+        const double g = attempt ? a.Winv[i * JOB_NE + col] : gam[t][e];
+        acc[t][e] = (i < R && col < R) ? g + job_reg(a, i, col, ridge) : (i == col ? 1.0 : 0.0);
       }
     TENRSO_JOB_MARK(3);
-    ok = job_block_gj(av, lds);
+    ok = job_block_gj(acc, lds);
     TENRSO_JOB_MARK(4);
     path = attempt;
   }
   if (!ok) path = 2;
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) a.Winv[(4 * bi + u) * JOB_NE + 4 * bj + v] = ok ? av[u][v] : 0.0;
+    for (int e = 0; e < 4; ++e) a.Winv[job_row(e) * JOB_NE + job_col(t)] = ok ? acc[t][e] : 0.0;
   TENRSO_JOB_MARK(5);
   if (threadIdx.x == 0 && a.scal) {
     const double prev = a.scal[a.status_slot];
