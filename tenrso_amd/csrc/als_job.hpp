@@ -32,6 +32,8 @@ constexpr int JOB_LDS_BYTES = (JOB_GJ_FLAG + 2) * 8;  // LDS bytes of the job
 
 // Device scalar block slots shared with the ALS driver.
 constexpr int JOB_SC_NORM_X = 0, JOB_SC_FIT = 1, JOB_SC_IP = 2, JOB_SC_RECON = 3, JOB_SC_STATUS = 4;
+constexpr int JOB_SC_STAMP = 7;  // host copy only: index of the sweep whose fit it holds
+constexpr int JOB_SC_COUNT = 8;  // size of the scalar block
 
 struct AlsJob {
   double* G;             // [nd][R][R] Grams (FP64)
@@ -47,6 +49,8 @@ struct AlsJob {
   double* scal;          // scalar block
   int status_slot;       // scal index that receives max(path)
   int status_reset;      // write path instead of max(previous, path)
+  double* host;          // with do_fit: mapped host copy of scal[0..JOB_SC_COUNT) (or null)
+  double stamp;          // written last into host[JOB_SC_STAMP]
 };
 
 __device__ __forceinline__ double job_recip(double p) {
@@ -330,6 +334,15 @@ __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
       a.scal[JOB_SC_FIT] = fit;
       a.scal[JOB_SC_IP] = ip;
       a.scal[JOB_SC_RECON] = recon;
+      if (a.host) {
+        // the scalar block as it stands (the solve status of the sweep just
+        // evaluated included) straight into host memory, then the stamp the
+        // host polls for: no copy launch, no event
+        for (int i = 0; i < JOB_SC_STAMP; ++i) a.host[i] = a.scal[i];
+        __threadfence_system();
+        a.host[JOB_SC_STAMP] = a.stamp;
+        __threadfence_system();
+      }
     }
     __syncthreads();
   }

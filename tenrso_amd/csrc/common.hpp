@@ -56,13 +56,17 @@ struct tenrso_ctx {
   int num_cus = 256;
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t ev_fit[2] = {nullptr, nullptr};  // "fit of sweep k is in pinned[k % 2]" markers
   double last_kernel_ms = 0.0;
   int profile_reps = 0;               // > 0: mttkrp_dense times its main kernel
   int wg_per_cu_f32 = 0;              // MTTKRP workgroups per CU; 0: the kernel's occupancy
   int wg_per_cu_bf16 = 0;             // (TENRSO_WG_PER_CU_F32 / _BF16 override)
   std::vector<tenrso::DevBuf> bufs;  // indexed by slot id
-  void* pinned = nullptr;            // small pinned host buffer for scalar readback
+  void* pinned = nullptr;            // small pinned (coherent) host buffer for scalar readback
+  double* pinned_dev = nullptr;      // the same buffer as a device pointer (kernels write the fit there)
+  // workgroups per CU of each MTTKRP kernel instance (occupancy query cache)
+  const void* occ_kern[32] = {};
+  int occ_val[32] = {};
+  int occ_n = 0;
 };
 
 namespace tenrso {

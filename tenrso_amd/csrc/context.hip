@@ -93,9 +93,8 @@ int tenrso_ctx_create(int device, int borrow_stream, void* stream, tenrso_ctx** 
     c->own_stream = true;
   }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_fit[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_fit[1], hipEventDisableTiming) != hipSuccess ||
-      hipHostMalloc(&c->pinned, 4096, hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc(&c->pinned, 4096, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&c->pinned_dev, c->pinned, 0) != hipSuccess) {
     set_error("context setup failed");
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -116,8 +115,6 @@ int tenrso_ctx_destroy(tenrso_ctx* c) {
     if (b.ptr) (void)hipFree(b.ptr);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  for (hipEvent_t e : c->ev_fit)
-    if (e) (void)hipEventDestroy(e);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -21,6 +21,7 @@
 // before it has read the fit of sweep k: if sweep k turns out to be the last,
 // sweep k + 1 has not touched its result.  The only host<->device traffic per
 // sweep is the scalar block copied back for the convergence test.
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -145,11 +146,29 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
     j.status_slot = SC_STATUS;
     return j;
   };
-  // copy the scalar block to pinned[k % 2] and mark it
-  auto publish = [&](int k) -> int {
-    TENRSO_HIP_TRY(hipMemcpyAsync(pinned + (k & 1) * SC_COUNT, scal, SC_COUNT * sizeof(double),
-                                  hipMemcpyDeviceToHost, ctx->stream));
-    TENRSO_HIP_TRY(hipEventRecord(ctx->ev_fit[k & 1], ctx->stream));
+  // The job evaluating the fit of sweep k writes the scalar block into
+  // pinned[k % 2] (mapped, coherent host memory) and then stamps it with k; the
+  // host polls the stamp.  (An event between the launches would cost the GPU
+  // an idle gap per sweep.)
+  static_assert(JOB_SC_COUNT == SC_COUNT && JOB_SC_STAMP < SC_COUNT, "scalar block layout");
+  auto host_block = [&](int k) { return ctx->pinned_dev + (k & 1) * SC_COUNT; };
+  pinned[JOB_SC_STAMP] = -1.0;
+  pinned[SC_COUNT + JOB_SC_STAMP] = -1.0;
+  auto wait_fit = [&](int k) -> int {
+    volatile double* stamp = pinned + (k & 1) * SC_COUNT + JOB_SC_STAMP;
+    const auto t_spin = std::chrono::steady_clock::now();
+    while (*stamp != (double)k) {
+      if (std::chrono::steady_clock::now() - t_spin > std::chrono::milliseconds(2)) {
+        // long sweep (or a failed launch): block on the stream, then the stamp must be there
+        TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (*stamp != (double)k) {
+          set_error("internal: the fit of sweep %d was not published", k);
+          return TENRSO_ERR_DEVICE;
+        }
+        break;
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
     return TENRSO_OK;
   };
   // rows of mode n, then their Gram
@@ -167,12 +186,15 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
       for (int m = 0; m < nd; ++m) Fc[m] = m < n ? news[m] : olds[m];
       AlsJob job = base_job();
       job.do_fit = (n == 0 && fit_prev) ? 1 : 0;
+      if (job.do_fit) {
+        job.host = host_block(k - 1);
+        job.stamp = (double)(k - 1);
+      }
       job.skip = n;
       job.status_slot = (k & 1) ? SC_STATUS_ODD : SC_STATUS;
       job.status_reset = n == 0 ? 1 : 0;
       MttkrpResult mr;
       TENRSO_TRY(pb.mttkrp(n, Fc, &job, &mr));
-      if (n == 0 && fit_prev) TENRSO_TRY(publish(k - 1));
       TENRSO_TRY(solve_and_gram(n, mr, news[n]));
     }
     return TENRSO_OK;
@@ -189,8 +211,9 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
     }
     AlsJob job = base_job();
     job.do_fit = 1;
-    TENRSO_TRY(als_job_launch(ctx, job));
-    return publish(k - 1);
+    job.host = host_block(k - 1);
+    job.stamp = (double)(k - 1);
+    return als_job_launch(ctx, job);
   };
 
   // speculation (queue sweep k before reading the fit of k - 1) keeps the GPU
@@ -215,7 +238,7 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
         const bool more = spec && k < o->max_iters;
         if (more) TENRSO_TRY(enqueue_sweep(k, true));
         else TENRSO_TRY(enqueue_fit(k));
-        TENRSO_HIP_TRY(hipEventSynchronize(ctx->ev_fit[(k - 1) & 1]));
+        TENRSO_TRY(wait_fit(k - 1));
         const double* hs = pinned + ((k - 1) & 1) * SC_COUNT;
         iters = k;
         const int it = k - 1;

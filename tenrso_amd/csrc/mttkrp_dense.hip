@@ -813,11 +813,24 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     // shares the launch, so every workgroup is resident at once
     int wpc = bf16 ? ctx->wg_per_cu_bf16 : ctx->wg_per_cu_f32;
     if (wpc <= 0) {
-      int nb = 0;
-      TENRSO_HIP_TRY(with_kernel([&](auto kern) {
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, WAVES * 64, 0);
-      }));
-      wpc = std::max(1, nb);
+      const void* kp = nullptr;
+      with_kernel([&](auto kern) {
+        kp = reinterpret_cast<const void*>(kern);
+        return 0;
+      });
+      for (int i = 0; i < ctx->occ_n; ++i)
+        if (ctx->occ_kern[i] == kp) wpc = ctx->occ_val[i];
+      if (wpc <= 0) {
+        int nb = 0;
+        TENRSO_HIP_TRY(with_kernel([&](auto kern) {
+          return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, WAVES * 64, 0);
+        }));
+        wpc = std::max(1, nb);
+        if (ctx->occ_n < 32) {
+          ctx->occ_kern[ctx->occ_n] = kp;
+          ctx->occ_val[ctx->occ_n++] = wpc;
+        }
+      }
     }
     const bool fuse_job = job != nullptr && rchunks == 1;
     const i64 target = (i64)wpc * ctx->num_cus - (fuse_job ? 1 : 0);
