@@ -458,12 +458,16 @@ void tikhonov(T* G, i64 n, T lam, i64 order) {
 
 // Householder thin QR, returns Q (rows x R) in place of Fm.  Stands in for
 // scirs2_linalg::qr in orthonormalize_factor (cp/helpers.rs:582-607); the sign
-// convention of the third-party routine is not visible (parity unpinned).
+// convention of the third-party routine is not visible (parity unpinned).  Q is
+// normalised to the QR factorisation with a positive diagonal of R (column k of
+// Q negated where the reflection made R[k][k] negative), the unique one for a
+// full-rank factor: Q = A chol(A^T A)^-T.
 template <class T>
 void orthonormalize(T* Fm, i64 m, i64 n) {
   std::vector<double> A((size_t)(m * n));
   for (i64 x = 0; x < m * n; ++x) A[(size_t)x] = Fm[x];
   std::vector<std::vector<double>> vs;
+  std::vector<double> rsign((size_t)n, 1.0);
   for (i64 k = 0; k < n && k < m; ++k) {
     std::vector<double> v((size_t)m, 0.0);
     double nrm = 0.0;
@@ -471,6 +475,7 @@ void orthonormalize(T* Fm, i64 m, i64 n) {
     nrm = std::sqrt(nrm);
     double a0 = A[(size_t)(k * n + k)];
     double alpha = a0 > 0 ? -nrm : nrm;
+    rsign[(size_t)k] = alpha < 0 ? -1.0 : 1.0;  // R[k][k] = alpha
     for (i64 i = k; i < m; ++i) v[(size_t)i] = A[(size_t)(i * n + k)];
     v[(size_t)k] -= alpha;
     double vn = 0.0;
@@ -500,7 +505,8 @@ void orthonormalize(T* Fm, i64 m, i64 n) {
       for (i64 i = k; i < m; ++i) Q[(size_t)(i * n + j)] -= d * v[(size_t)i];
     }
   }
-  for (i64 x = 0; x < m * n; ++x) Fm[x] = (T)Q[(size_t)x];
+  for (i64 i = 0; i < m; ++i)
+    for (i64 j = 0; j < n; ++j) Fm[i * n + j] = (T)(Q[(size_t)(i * n + j)] * rsign[(size_t)j]);
 }
 
 }  // namespace

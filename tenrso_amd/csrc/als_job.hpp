@@ -297,16 +297,21 @@ __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
       cross[t][e] = 1.0;
       gam[t][e] = 1.0;
     }
-  for (int k = 0; k < a.nd; ++k) {
+  // Gram blocks of modes k0, k0 + 1 loaded together (two round trips fewer for
+  // three modes than one mode at a time; the MTTKRP beside the job keeps the
+  // memory system busy, so each round trip is long)
+  auto load_block = [&](int k, double (&g)[4][4]) {
     const double* Gk = a.G + (i64)k * R * R;
-    double g[4][4];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int i = job_row(e), j = job_col(t);
-        g[t][e] = (i < R && j < R) ? Gk[i * R + j] : 0.0;
+        g[t][e] = (k < a.nd && i < R && j < R) ? Gk[i * R + j] : 0.0;
       }
+  };
+  auto fold = [&](int k, const double (&g)[4][4]) {
+    if (k >= a.nd) return;
     const bool in_gam = k != a.skip;
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -315,6 +320,13 @@ __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
         cross[t][e] *= g[t][e];
         gam[t][e] = in_gam ? gam[t][e] * g[t][e] : gam[t][e];
       }
+  };
+  for (int k = 0; k < a.nd; k += 2) {
+    double g0[4][4], g1[4][4];
+    load_block(k, g0);
+    load_block(k + 1, g1);
+    fold(k, g0);
+    fold(k + 1, g1);
   }
   if (want_fit) {
     // ||Xhat||^2 = sum_{r,s} prod_k G_k[r,s]; <X, Xhat> = sum of the partials
@@ -357,17 +369,23 @@ __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) a.Winv[job_row(e) * JOB_NE + job_col(t)] = gam[t][e];
   doublex4 acc[4];
-#pragma unroll 1
-  for (int attempt = 0; attempt < 2 && !ok; ++attempt) {
-    const double ridge = attempt ? (double)FLT_EPSILON * (double)(10 * R) : 0.0;
+  // Gamma + regularisation, identity outside R x R; the retry (ridge) reads Gamma
+  // back from Winv, so gam is dead once the first attempt's matrix is formed
+  auto form = [&](bool retry) {
+    const double ridge = retry ? (double)FLT_EPSILON * (double)(10 * R) : 0.0;
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int i = job_row(e), col = job_col(t);
-        const double g = attempt ? a.Winv[i * JOB_NE + col] : gam[t][e];
+        const double g = retry ? a.Winv[i * JOB_NE + col] : gam[t][e];
         acc[t][e] = (i < R && col < R) ? g + job_reg(a, i, col, ridge) : (i == col ? 1.0 : 0.0);
       }
+  };
+  form(false);
+#pragma unroll 1
+  for (int attempt = 0; attempt < 2 && !ok; ++attempt) {
+    if (attempt) form(true);
     TENRSO_JOB_MARK(3);
     ok = job_block_gj(acc, lds);
     TENRSO_JOB_MARK(4);

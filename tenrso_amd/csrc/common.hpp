@@ -87,6 +87,7 @@ enum BufSlot : int {
   SLOT_TMP2,
   SLOT_IP,
   SLOT_SPM,
+  SLOT_ORTHO,
   SLOT_FSET,
   SLOT_COUNT
 };

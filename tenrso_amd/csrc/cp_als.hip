@@ -68,10 +68,6 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
     set_error("rank %lld exceeds the on-device normal-equation solve bound %d", (long long)R, JOB_NE);
     return TENRSO_ERR_UNSUPPORTED;
   }
-  if (o->orthogonal) {
-    set_error("orthogonal constraint is not supported by the device CP-ALS");
-    return TENRSO_ERR_UNSUPPORTED;
-  }
   TENRSO_TRY(set_device(ctx));
   i64 maxI = 0, ftot = 0;
   for (int k = 0; k < nd; ++k) {
@@ -91,7 +87,10 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
   double* Winv = (double*)p_Winv;
   double* ip_part = (double*)p_tmp;
   double* M = nullptr;
-  if (o->fit_from_mode0) {
+  // the orthogonal constraint changes A_{N-1} after its solve, so the fit takes
+  // <X, Xhat> from a mode-0 MTTKRP of the final factors, as the reference does
+  const bool fit0 = o->fit_from_mode0 || o->orthogonal;
+  if (fit0) {
     void* p_M;
     TENRSO_TRY(ctx_buffer(ctx, SLOT_M64, (size_t)maxI * R * sizeof(double), &p_M));
     M = (double*)p_M;
@@ -174,6 +173,8 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
   // rows of mode n, then their Gram
   auto solve_and_gram = [&](int n, const MttkrpResult& mr, float* An) -> int {
     TENRSO_TRY(solve_rows(ctx, mr.partial, mr.splits, nullptr, shape[n], R, Winv, epi, An, ip_part, &nblk));
+    // orthogonal constraint (core.rs:362-365): A_n <- Q of its QR
+    if (o->orthogonal) TENRSO_TRY(orthonormalize_rows(ctx, An, shape[n], R));
     TENRSO_TRY(gram_f64(ctx, An, shape[n], R, G + (i64)n * R * R));
     return TENRSO_OK;
   };
@@ -202,7 +203,7 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
   // fit of sweep k - 1 on its own (no further sweep queued behind it)
   auto enqueue_fit = [&](int k) -> int {
     float* const* cur = set[k & 1];
-    if (o->fit_from_mode0) {
+    if (fit0) {
       // cp/helpers.rs:558-579: <X, Xhat> from a mode-0 MTTKRP with every updated factor
       MttkrpResult mr;
       TENRSO_TRY(pb.mttkrp(0, (const float* const*)cur, nullptr, &mr));
@@ -219,7 +220,7 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
   // speculation (queue sweep k before reading the fit of k - 1) keeps the GPU
   // busy through the host's convergence test; with a time limit or the
   // reference fit form each sweep is queued only after that test.
-  const bool spec = !o->fit_from_mode0 && !(o->time_limit_s >= 0.0);
+  const bool spec = !fit0 && !(o->time_limit_s >= 0.0);
   double prev_fit = 0.0, fit = 0.0, final_change = 0.0;
   int iters = 0, osc = 0, reason = TENRSO_REASON_MAX_ITERATIONS, worst = 0;
   const auto t0 = std::chrono::steady_clock::now();
@@ -278,6 +279,7 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
   }
   TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));  // drains a speculative sweep
   if (status != TENRSO_OK) return status;
+
   // the result of the last committed sweep (iters - 1) is in set iters % 2
   if ((iters & 1) == 1) {
     for (int k = 0; k < nd; ++k)

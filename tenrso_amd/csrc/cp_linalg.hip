@@ -231,6 +231,123 @@ int solve_rows(tenrso_ctx* ctx, const float* part, int splits, const double* M, 
   return TENRSO_OK;
 }
 
+// ---------------------------------------------------------------- orthonormalisation
+// Q of the QR factorisation of a rows x R fp32 factor (orthonormalize_factor,
+// cp/helpers.rs:582-607), one workgroup, FP64: Householder reflections
+// H_k = I - 2 v v^T / (v^T v), v = x - alpha e_k, alpha = -sign(x_k) ||x||
+// (the oracle's restatement), then Q = H_0 ... H_{R-1} [I; 0], with column k
+// negated where alpha < 0 so that R has a positive diagonal.  A column that is
+// already zero below and on the diagonal takes no reflection (rank-deficient
+// factors get an orthonormal completion, as Householder QR gives).
+// W: 2 * rows * R doubles of scratch (the reduced matrix with the reflectors
+// below its diagonal, then Q).  Thread t works on column t % 64 and row slice
+// t / 64 (4 slices); partial dot products are added in slice order.
+__global__ __launch_bounds__(256) void householder_q_kernel(float* __restrict__ A, i64 rows, int R,
+                                                            double* __restrict__ W) {
+  __shared__ double red[4][NE];
+  __shared__ double dots[NE];
+  __shared__ double v0s[NE], vns[NE], sgn[NE];
+  const int tid = threadIdx.x, col = tid & 63, slice = tid >> 6;
+  double* Q = W + rows * R;
+  const i64 n = rows * R;
+  for (i64 e = tid; e < n; e += 256) W[e] = (double)A[e];
+  __syncthreads();
+  const int kmax = (int)(rows < R ? rows : R);
+  // row range of this slice within [lo, rows)
+  auto slice_rows = [&](i64 lo, i64& r0, i64& r1) {
+    const i64 len = rows - lo;
+    r0 = lo + len * slice / 4;
+    r1 = lo + len * (slice + 1) / 4;
+  };
+  for (int k = 0; k < kmax; ++k) {
+    i64 r0, r1;
+    slice_rows(k, r0, r1);
+    // ||x||^2, x = W[k:, k]
+    double p = 0.0;
+    if (col == 0)
+      for (i64 i = r0; i < r1; ++i) p = fma(W[i * R + k], W[i * R + k], p);
+    if (col == 0) red[slice][0] = p;
+    __syncthreads();
+    const double nrm2 = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+    const double a0 = W[(i64)k * R + k];
+    const double nrm = sqrt(nrm2);
+    const double alpha = a0 > 0.0 ? -nrm : nrm;
+    const double v0 = a0 - alpha;
+    const double vn = (nrm2 - a0 * a0) + v0 * v0;
+    if (tid == 0) {
+      v0s[k] = v0;
+      vns[k] = vn;
+      sgn[k] = alpha < 0.0 ? -1.0 : 1.0;
+    }
+    // d_j = v^T W[:, j] for j > k (v_k = v0, v_i = W[i][k] below)
+    double d = 0.0;
+    if (vn > 0.0 && col > k && col < R) {
+      for (i64 i = r0; i < r1; ++i) {
+        const double vi = (i == k) ? v0 : W[i * R + k];
+        d = fma(vi, W[i * R + col], d);
+      }
+    }
+    __syncthreads();  // red[.][0] read by everyone before it is reused
+    red[slice][col] = d;
+    __syncthreads();
+    if (slice == 0) dots[col] = 2.0 * ((red[0][col] + red[1][col]) + (red[2][col] + red[3][col])) / (vn > 0.0 ? vn : 1.0);
+    __syncthreads();
+    if (vn > 0.0 && col > k && col < R) {
+      const double dj = dots[col];
+      for (i64 i = r0; i < r1; ++i) {
+        const double vi = (i == k) ? v0 : W[i * R + k];
+        W[i * R + col] -= dj * vi;
+      }
+    }
+    __syncthreads();
+  }
+  // Q = H_0 ... H_{kmax-1} applied to the first R columns of the identity
+  for (i64 e = tid; e < n; e += 256) Q[e] = (e / R == e % R) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int k = kmax - 1; k >= 0; --k) {
+    const double vn = vns[k], v0 = v0s[k];
+    if (!(vn > 0.0)) continue;  // uniform across the workgroup
+    i64 r0, r1;
+    slice_rows(k, r0, r1);
+    double d = 0.0;
+    if (col >= k && col < R) {
+      for (i64 i = r0; i < r1; ++i) {
+        const double vi = (i == k) ? v0 : W[i * R + k];
+        d = fma(vi, Q[i * R + col], d);
+      }
+    }
+    red[slice][col] = d;
+    __syncthreads();
+    if (slice == 0) dots[col] = 2.0 * ((red[0][col] + red[1][col]) + (red[2][col] + red[3][col])) / vn;
+    __syncthreads();
+    if (col >= k && col < R) {
+      const double dj = dots[col];
+      for (i64 i = r0; i < r1; ++i) {
+        const double vi = (i == k) ? v0 : W[i * R + k];
+        Q[i * R + col] -= dj * vi;
+      }
+    }
+    __syncthreads();
+  }
+  for (i64 e = tid; e < n; e += 256) {
+    const int j = (int)(e % R);
+    A[e] = (float)(Q[e] * (j < kmax ? sgn[j] : 1.0));
+  }
+}
+
+int orthonormalize_rows(tenrso_ctx* ctx, float* A, i64 rows, i64 R) {
+  if (R > SOLVE_RMAX) {
+    set_error("rank %lld exceeds the on-device orthonormalisation bound %d", (long long)R, SOLVE_RMAX);
+    return TENRSO_ERR_UNSUPPORTED;
+  }
+  if (rows == 0 || R == 0) return TENRSO_OK;
+  void* pw = nullptr;
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_ORTHO, (size_t)2 * rows * R * sizeof(double), &pw));
+  hipLaunchKernelGGL(householder_q_kernel, dim3(1), dim3(256), 0, ctx->stream, A, rows, (int)R, (double*)pw);
+  TENRSO_HIP_TRY(hipGetLastError());
+  return TENRSO_OK;
+}
+
 __global__ __launch_bounds__(256) void inner_rows_kernel(const double* __restrict__ M, const float* __restrict__ A,
                                                          i64 rows, i64 R, double* __restrict__ ip_part) {
   __shared__ double red[256];

@@ -39,6 +39,9 @@ int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G, hipStr
 // (or the fp64 M when part is null); ip_part[blk] = sum_i M[i,:].A[i,:] (FP64).
 int solve_rows(tenrso_ctx* ctx, const float* part, int splits, const double* M, i64 rows, i64 R,
                const double* Winv, const RowEpilogue& epi, float* A, double* ip_part, int* nblk_out);
+// A <- Q of the QR factorisation of the rows x R factor A (Householder, FP64,
+// one workgroup), normalised to a positive diagonal of R.
+int orthonormalize_rows(tenrso_ctx* ctx, float* A, i64 rows, i64 R);
 // ip_part[blk] = sum_{i in blk} M[i,:] . A[i,:]
 int inner_rows(tenrso_ctx* ctx, const double* M, const float* A, i64 rows, i64 R, double* ip_part, int* nblk_out);
 // x = max(x, 0) on an fp32 array.

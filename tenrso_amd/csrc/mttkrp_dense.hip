@@ -808,32 +808,36 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
       if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES, 2, 1, KWMAX>);
       return f(mttkrp_bf16_mfma<4, WAVES, 2, 1, KWMAX>);
     };
-    // workgroups per CU: what the instance can keep resident (registers, LDS),
-    // unless TENRSO_WG_PER_CU_* overrides it; one slot fewer when the ALS job
-    // shares the launch, so every workgroup is resident at once
+    // workgroups per CU doing MTTKRP work: 2 (the sweet spot swept at 256^3 and
+    // 512x512x128), fewer if the instance cannot keep 2 resident, or the
+    // TENRSO_WG_PER_CU_* override.  When the ALS job shares the launch it takes a
+    // third resident slot on one CU if the instance fits 3, otherwise one of the
+    // MTTKRP slots -- either way every workgroup is resident at once.
     int wpc = bf16 ? ctx->wg_per_cu_bf16 : ctx->wg_per_cu_f32;
-    if (wpc <= 0) {
+    int occ = 0;
+    {
       const void* kp = nullptr;
       with_kernel([&](auto kern) {
         kp = reinterpret_cast<const void*>(kern);
         return 0;
       });
       for (int i = 0; i < ctx->occ_n; ++i)
-        if (ctx->occ_kern[i] == kp) wpc = ctx->occ_val[i];
-      if (wpc <= 0) {
+        if (ctx->occ_kern[i] == kp) occ = ctx->occ_val[i];
+      if (occ <= 0) {
         int nb = 0;
         TENRSO_HIP_TRY(with_kernel([&](auto kern) {
           return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, WAVES * 64, 0);
         }));
-        wpc = std::max(1, nb);
+        occ = std::max(1, nb);
         if (ctx->occ_n < 32) {
           ctx->occ_kern[ctx->occ_n] = kp;
-          ctx->occ_val[ctx->occ_n++] = wpc;
+          ctx->occ_val[ctx->occ_n++] = occ;
         }
       }
     }
+    if (wpc <= 0) wpc = std::min(occ, 2);
     const bool fuse_job = job != nullptr && rchunks == 1;
-    const i64 target = (i64)wpc * ctx->num_cus - (fuse_job ? 1 : 0);
+    const i64 target = (i64)wpc * ctx->num_cus - ((fuse_job && occ <= wpc) ? 1 : 0);
     // each workgroup keeps >= WAVES slow indices
     i64 ssp = std::max<i64>(1, target / std::max<i64>(1, tiles) / a.nwin);
     ssp = std::min<i64>(ssp, a.n_slow >= WAVES ? a.n_slow / WAVES : 1);

@@ -193,6 +193,49 @@ def test_cp_als_host_path_and_errors():
     assert cp0.iters == 0 and cp0.convergence.reason == decomp.ConvergenceReason.TimeLimit
 
 
+CONSTRAINTS = [
+    ("nonnegative", dict(nonnegative=True), lambda: decomp.CpConstraints.nonnegative_()),
+    ("l2", dict(l2=0.5), lambda: decomp.CpConstraints.l2_regularized(0.5)),
+    ("l1", dict(l1=0.05), lambda: decomp.CpConstraints.l1_regularized(0.05)),
+    ("tikhonov1", dict(tikhonov_order=1, tikhonov_lambda=0.3), lambda: decomp.CpConstraints.tikhonov(0.3, 1)),
+    ("tikhonov2", dict(tikhonov_order=2, tikhonov_lambda=0.3), lambda: decomp.CpConstraints.tikhonov(0.3, 2)),
+    ("orthogonal", dict(orthogonal=True), lambda: decomp.CpConstraints.orthogonal_()),
+    ("nonneg+orthogonal", dict(nonnegative=True, orthogonal=True),
+     lambda: decomp.CpConstraints(nonnegative=True, orthogonal=True)),
+]
+
+
+@pytest.mark.parametrize("name,orc_kw,make", CONSTRAINTS, ids=[c[0] for c in CONSTRAINTS])
+def test_cp_als_constrained_vs_oracle(name, orc_kw, make):
+    # cp_als_constrained (core.rs:225-408) on 48x40x36 rank 6 from the same init:
+    # fit history within 1e-4 of the FP64 oracle, same iteration count and reason,
+    # factors within 1e-3.  The data is nonnegative so the input check passes.
+    # Orthogonal: Q of the positive-diagonal QR on both sides (the third-party
+    # routine's sign convention is not visible); the factors must come out with
+    # orthonormal columns.
+    print(f"[case] cp_als constrained {name} 48x40x36 rank 6", flush=True)
+    rng = np.random.default_rng(31)
+    shape, R = (48, 40, 36), 6
+    A = [rng.random((n, R)) for n in shape]
+    X = (np.einsum("ir,jr,kr->ijk", *A) + 0.01 * rng.random(shape)).astype(np.float32)
+    init = [rng.random((n, R)).astype(np.float32) + 0.1 for n in shape]
+    cp = decomp.cp_als_with_init(dev(X), [dev(f) for f in init], 15, 1e-6, constraints=make())
+    ref_F, info = oracle.cp_als(X, init, 15, 1e-6, **orc_kw)
+    errs = [rel(g.cpu().numpy(), r) for g, r in zip(cp.factors, ref_F)]
+    print(f"  gpu fit {cp.fit:.8f} iters {cp.iters} {cp.convergence.reason.name}; oracle fit {info['fit']:.8f} "
+          f"iters {info['iters']} {info['reason']}; factor rel errs {['%.1e' % e for e in errs]}", flush=True)
+    assert abs(cp.fit - info["fit"]) <= 1e-4
+    assert cp.iters == info["iters"] and cp.convergence.reason.name == info["reason"]
+    assert np.abs(np.array(cp.convergence.fit_history) - info["fit_history"]).max() <= 1e-4
+    assert max(errs) < 1e-3
+    if orc_kw.get("orthogonal"):
+        for g in cp.factors:
+            g = g.cpu().numpy().astype(np.float64)
+            assert np.allclose(g.T @ g, np.eye(R), atol=1e-5)
+    if orc_kw.get("nonnegative") and not orc_kw.get("orthogonal"):
+        assert all(float(g.min()) >= 0.0 for g in cp.factors)
+
+
 def test_cp_als_bf16_fit_close():
     print("[case] cp_als bf16 256^3 rank 64, 10 iters", flush=True)
     X, _, _, init = synth.dense_256_r64()

@@ -311,3 +311,19 @@ def test_coo_mttkrp_matches_dense(orc):
     F = [rng.standard_normal((n, 4)) for n in shape]
     for mode in range(3):
         assert np.allclose(orc.coo_mttkrp(idx, vals, shape, F, mode), einsum_mttkrp(dense.astype(np.float64), F, mode))
+
+
+def test_orthonormalize_is_the_positive_diagonal_qr(orc):
+    # Q of A = QR with R upper triangular and positive diagonal: Q^T Q = I,
+    # Q^T A upper triangular with positive diagonal, and the closed form
+    # Q = A chol(A^T A)^-T.  A sign slip in one reflection fails the second check.
+    rng = np.random.default_rng(21)
+    for m, n in [(9, 4), (40, 16), (256, 64)]:
+        A = rng.standard_normal((m, n))
+        Q = orc.orthonormalize(A)
+        assert np.allclose(Q.T @ Q, np.eye(n), atol=1e-12)
+        Rm = Q.T @ A
+        assert np.allclose(np.tril(Rm, -1), 0.0, atol=1e-10)
+        assert np.all(np.diag(Rm) > 0)
+        L = np.linalg.cholesky(A.T @ A)
+        assert np.allclose(Q, A @ np.linalg.inv(L).T, atol=1e-9)
