@@ -39,6 +39,8 @@ namespace tenrso {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
 
 struct KernArgs {
   i64 I, Q;           // X viewed as [P, I, Q]
@@ -51,9 +53,11 @@ struct KernArgs {
   int nwin;           // fast_size / kw
   i64 n_slow;         // J / fast_size
   int slow_splits;    // splits of the slow range; grid.y = nwin * slow_splits
+  i64 slow_q, slow_rem;  // n_slow = slow_q * slow_splits + slow_rem
   int r0;             // first rank column of this launch
   int rcount;         // rank columns handled by the launch (<= 16*NT)
   int rtiles;         // row tiles; workgroup tile -> (tile % rtiles, split = tile / rtiles)
+  int win_vec;        // FAST 16-byte aligned and R % 4 == 0: 16-byte window loads
 };
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
@@ -64,79 +68,195 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
 }
 
 // Stage FAST[w*kw .. w*kw+kw) (columns r0 .. r0+RT) into LDS transposed: s[r][kk].
-// Columns past rcount are zero, so they contribute nothing.  Every thread issues
-// all of its loads before its first LDS write (one memory round trip).
-template <int RT, int PITCH, int NTH, int KWMAX>
-__device__ __forceinline__ void stage_fast_window(float* __restrict__ s, const KernArgs& a, int w) {
-  constexpr int PER = (RT * KWMAX + NTH - 1) / NTH;
-  const i64 k0 = (i64)w * a.kw;
-  const int n = RT * a.kw;
-  float v[PER];
+// Work unit u of the workgroup covers ranks r4 .. r4+3 (r4 = 4*(u % (RT/4))) of
+// window rows kk = 2*(u / (RT/4)) and kk + 1, so it leaves LDS as one 8-byte
+// (fp32) or two 4-byte (bf16 hi, lo) pairs per rank.  With a 16-byte aligned
+// FAST and R % 4 == 0 (win_vec) a unit is two 16-byte loads, else eight 4-byte
+// loads (one kernel instance per case, so that every instance has a single
+// load sequence and the waits stay exact).  Ranks past rcount and rows past kw load column r0 of window row 0
+// instead (always inside FAST): those rank columns are never stored and those
+// rows never read, so no value select is needed and no load waits early.
+// Split in two so the window loads are issued before the first X blocks: the
+// LDS writes then wait for the window only (vector loads retire in issue order).
+template <int RT, int NTH, int KWMAX>
+struct FastWindow {
+  static constexpr int UNITS = RT / 4 * (KWMAX / 2);
+  static constexpr int PERU = UNITS / NTH;
+  static_assert(UNITS % NTH == 0, "window units must split evenly over the workgroup");
+  floatx4 v[PERU][2];
+};
+
+template <bool WV, int RT, int NTH, int KWMAX>
+__device__ __forceinline__ void stage_window_load(FastWindow<RT, NTH, KWMAX>& fw, const KernArgs& a, int w) {
+  const float* base = a.fast + (i64)w * a.kw * a.R + a.r0;
+  if constexpr (WV) {
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = threadIdx.x + e * NTH;
-    const int r = idx % RT, kk = idx / RT;
-    v[e] = (idx < n && r < a.rcount) ? a.fast[(k0 + kk) * a.R + a.r0 + r] : 0.0f;
+    for (int e = 0; e < FastWindow<RT, NTH, KWMAX>::PERU; ++e) {
+      const int u = threadIdx.x + e * NTH;
+      const int r4 = 4 * (u % (RT / 4)), kk = 2 * (u / (RT / 4));
+      const bool ok = kk < a.kw && r4 < a.rcount;
+      const float* p = base + (ok ? kk * a.R + r4 : 0);
+      fw.v[e][0] = *reinterpret_cast<const floatx4*>(p);
+      fw.v[e][1] = *reinterpret_cast<const floatx4*>(p + (ok ? a.R : 0));
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < FastWindow<RT, NTH, KWMAX>::PERU; ++e) {
+      const int u = threadIdx.x + e * NTH;
+      const int r4 = 4 * (u % (RT / 4)), kk = 2 * (u / (RT / 4));
+      const bool rows_ok = kk < a.kw;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const bool ok = rows_ok && r4 + q < a.rcount;
+        const float* p = base + (ok ? kk * a.R + r4 + q : 0);
+        fw.v[e][0][q] = p[0];
+        fw.v[e][1][q] = p[ok ? a.R : 0];
+      }
+    }
   }
+}
+
+template <int RT, int PITCH, int NTH, int KWMAX>
+__device__ __forceinline__ void stage_window_store(float* __restrict__ s, const FastWindow<RT, NTH, KWMAX>& fw) {
+  static_assert(PITCH % 2 == 0, "pairs of window rows are stored as 8-byte words");
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = threadIdx.x + e * NTH;
-    if (idx < n) s[(idx % RT) * PITCH + idx / RT] = v[e];
+  for (int e = 0; e < FastWindow<RT, NTH, KWMAX>::PERU; ++e) {
+    const int u = threadIdx.x + e * NTH;
+    const int r4 = 4 * (u % (RT / 4)), kk = 2 * (u / (RT / 4));
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      *reinterpret_cast<float2*>(&s[(r4 + q) * PITCH + kk]) = float2{fw.v[e][0][q], fw.v[e][1][q]};
   }
 }
 
 // Same window as an exact bf16 pair: FAST = hi + lo with hi = bf16(FAST) and
 // lo = bf16(FAST - hi), so hi + lo carries 16 significant bits.
 template <int RT, int PITCH, int NTH, int KWMAX>
-__device__ __forceinline__ void stage_fast_window_bf16x2(uint16_t* __restrict__ shi, uint16_t* __restrict__ slo,
-                                                         const KernArgs& a, int w) {
-  constexpr int PER = (RT * KWMAX + NTH - 1) / NTH;
-  const i64 k0 = (i64)w * a.kw;
-  const int n = RT * a.kw;
-  float v[PER];
+__device__ __forceinline__ void stage_window_store_bf16x2(uint16_t* __restrict__ shi, uint16_t* __restrict__ slo,
+                                                          const FastWindow<RT, NTH, KWMAX>& fw) {
+  static_assert(PITCH % 2 == 0, "pairs of window rows are stored as 4-byte words");
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = threadIdx.x + e * NTH;
-    const int r = idx % RT, kk = idx / RT;
-    v[e] = (idx < n && r < a.rcount) ? a.fast[(k0 + kk) * a.R + a.r0 + r] : 0.0f;
-  }
+  for (int e = 0; e < FastWindow<RT, NTH, KWMAX>::PERU; ++e) {
+    const int u = threadIdx.x + e * NTH;
+    const int r4 = 4 * (u % (RT / 4)), kk = 2 * (u / (RT / 4));
 #pragma unroll
-  for (int e = 0; e < PER; ++e) {
-    const int idx = threadIdx.x + e * NTH;
-    if (idx < n) {
-      const int o = (idx % RT) * PITCH + idx / RT;
-      const uint16_t h = f32_to_bf16_rne(v[e]);
-      shi[o] = h;
-      slo[o] = f32_to_bf16_rne(v[e] - __uint_as_float((uint32_t)h << 16));
+    for (int q = 0; q < 4; ++q) {
+      // v_cvt_pk_bf16_f32 rounds to nearest even, as f32_to_bf16_rne does for
+      // every non-NaN value; element 0 (row kk) is the low half
+      const floatx2 f = {fw.v[e][0][q], fw.v[e][1][q]};
+      const uint32_t hw = __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2));
+      const floatx2 r = {f[0] - __uint_as_float(hw << 16), f[1] - __uint_as_float(hw & 0xffff0000u)};
+      const uint32_t lw = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
+      const int o = (r4 + q) * PITCH + kk;
+      *reinterpret_cast<uint32_t*>(&shi[o]) = hw;
+      *reinterpret_cast<uint32_t*>(&slo[o]) = lw;
     }
   }
 }
 
-// Fixed-order reduction of the WAVES per-wave accumulators of a workgroup into
-// one [ROWS x RT] partial (waves 1.. stage through LDS, wave 0 adds in order).
+// A value the compiler cannot prove uniform but that is: move it to scalar registers.
+__device__ __forceinline__ i64 uniform_i64(i64 v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (i64)(((uint64_t)hi << 32) | lo);
+}
+
+// PRE row used when the MTTKRP has no PRE factor (a matrix): KR = FAST.
+#define TENRSO_ONES8 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f
+__device__ float kPreOnes[64] = {TENRSO_ONES8, TENRSO_ONES8, TENRSO_ONES8, TENRSO_ONES8,
+                                       TENRSO_ONES8, TENRSO_ONES8, TENRSO_ONES8, TENRSO_ONES8};
+#undef TENRSO_ONES8
+
+// Block schedule of one wave.  Block t of the wave is block kb = t % nkb of the
+// slow index s_lo + wave + WAVES*(t / nkb).  Loads run in t order, so the cursor
+// only steps forward, and it stops on the wave's last block: loads past the end
+// re-read that block (a cache hit).  Every X and PRE load is thus unconditional
+// and the wait before a block's MFMAs counts exactly the loads issued after it,
+// which keeps the prefetch in flight.  A wave with no block reads the block of
+// slow index min(s_lo + wave, n_slow - 1), always inside X, and computes nothing.
+// Case A (X as [P, I, Q], slow = p*qs + sq): the block starts at element
+// p*I*Q + sq*fast_size + w*kw + kb*KB.  Case B: the block covers j = slow*fast_size
+// + w*kw + kb*KB onwards, one row of I elements per j.  All fields are uniform.
+template <int WAVES, int KB, bool CASE_B>
+struct BlockCursor {
+  int kb, nkb;
+  i64 slow, sq, base, wrap;
+
+  __device__ __forceinline__ BlockCursor(const KernArgs& a, i64 slow0, int w, int nkb_) {
+    kb = 0;
+    nkb = nkb_;
+    slow = slow0 < a.n_slow ? slow0 : a.n_slow - 1;
+    if constexpr (!CASE_B) {
+      // 32-bit division whenever the slow range fits (it nearly always does)
+      const i64 p = (a.n_slow >> 32) ? slow / a.qs : (i64)((uint32_t)slow / (uint32_t)a.qs);
+      sq = slow - p * a.qs;
+      base = p * a.I * a.Q + sq * a.fast_size + (i64)w * a.kw;
+      wrap = a.I * a.Q - a.qs * a.fast_size;  // from (p, qs) to (p + 1, 0)
+    } else {
+      sq = 0;
+      wrap = 0;
+      base = slow * a.fast_size + (i64)w * a.kw;
+    }
+  }
+  // element offset in X of the block's first element (lane offsets are added on top)
+  __device__ __forceinline__ i64 offset(const KernArgs& a) const {
+    if constexpr (!CASE_B) return base + (i64)kb * KB;
+    else return (base + (i64)kb * KB) * a.I;
+  }
+  __device__ __forceinline__ void step(const KernArgs& a) {
+    if (++kb == nkb) {
+      kb = 0;
+      slow += WAVES;
+      base += (i64)WAVES * a.fast_size;
+      if constexpr (!CASE_B) {
+        sq += WAVES;
+        while (sq >= a.qs) {
+          sq -= a.qs;
+          base += wrap;
+        }
+      }
+    }
+  }
+};
+
+// Fixed-order reduction of the WAVES per-wave accumulators of a workgroup, spread
+// over the waves: tile q = mt*NT + nt belongs to wave q % WAVES, which adds the
+// WAVES copies of it in wave order 0, 1, .. (one LDS round trip, 16 bytes per
+// lane and tile) and stores it; every wave stores its own tiles, so the partial
+// leaves the workgroup from all waves at once.  LDS use: MT*NT*WAVES*64 floatx4.
 template <int MT, int NT, int WAVES>
-__device__ __forceinline__ void wg_reduce_acc(floatx4 (&acc)[MT][NT], float* __restrict__ red, int wave, int lane) {
-  constexpr int PER = MT * NT * 4;  // floats per lane
-  __syncthreads();                  // LDS window no longer read
-  if (wave > 0) {
+__device__ __forceinline__ void wg_reduce_store(floatx4 (&acc)[MT][NT], float* __restrict__ red, int wave, int lane,
+                                                float* __restrict__ out, i64 rows, i64 R, int r0, int rcount,
+                                                i64 row_base, bool case_b) {
+  floatx4* r4 = reinterpret_cast<floatx4*>(red);
+  const int g = lane >> 4, c = lane & 15;
+  __syncthreads();  // LDS window no longer read
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) red[((wave - 1) * PER + (mt * NT + nt) * 4 + v) * 64 + lane] = acc[mt][nt][v];
-  }
+    for (int nt = 0; nt < NT; ++nt) {
+      const int q = mt * NT + nt;
+      if (q % WAVES != wave) r4[(q * WAVES + wave) * 64 + lane] = acc[mt][nt];
+    }
   __syncthreads();
-  if (wave == 0) {
 #pragma unroll
-    for (int wv = 1; wv < WAVES; ++wv)
+  for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
+    for (int nt = 0; nt < NT; ++nt) {
+      const int q = mt * NT + nt;
+      if (q % WAVES != wave) continue;
+      floatx4 sum = wave == 0 ? acc[mt][nt] : r4[(q * WAVES + 0) * 64 + lane];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
+      for (int wv = 1; wv < WAVES; ++wv) sum += wv == wave ? acc[mt][nt] : r4[(q * WAVES + wv) * 64 + lane];
+      if (16 * nt + c >= rcount) continue;
+      const int col = r0 + 16 * nt + c;
+      // D layout (16x16): lane (g, c), register v -> tile row 4g + v, column c
 #pragma unroll
-          for (int v = 0; v < 4; ++v) acc[mt][nt][v] += red[((wv - 1) * PER + (mt * NT + nt) * 4 + v) * 64 + lane];
-  }
+      for (int v = 0; v < 4; ++v) {
+        const i64 row = case_b ? row_base + MT * (4 * g + v) + mt : row_base + 16 * mt + 4 * g + v;
+        if (row < rows) out[row * R + col] = sum[v];
+      }
+    }
 }
 
 // Fold the per-slow accumulator T into acc with the PRE row of that slow index:
@@ -170,14 +290,15 @@ __device__ __forceinline__ void fold_slow(floatx4 (&acc)[MT][NT], floatx4 (&T)[M
 //   Case B (i contiguous, last mode): rows are interleaved, lane c of m-tile mt
 //     holds row row_base + MT*c + mt, so one MT-wide vector load per j covers
 //     16*MT consecutive rows.
-//   X blocks are prefetched 3 blocks ahead in registers; the first loads are
-//   issued before the LDS window is staged.
+//   X blocks are prefetched 3 blocks ahead in registers.  The FAST window loads
+//   are issued first, then the first three X blocks, so the window reaches LDS
+//   while X is still arriving.
 // ---------------------------------------------------------------------------
-template <int NT, int WAVES, int MT, bool CASE_B, int KWMAX>
+template <int NT, int WAVES, int MT, bool CASE_B, int KWMAX, bool WV>
 __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __restrict__ X, KernArgs a,
                                                               float* __restrict__ partial, AlsJob job, int has_job) {
   constexpr int KB = 32, PITCH = KWMAX + 4, RT = 16 * NT, NTH = WAVES * 64, ROWS = 16 * MT;
-  constexpr int LDS_WIN = RT * PITCH, LDS_RED = (WAVES - 1) * MT * NT * 4 * 64;
+  constexpr int LDS_WIN = RT * PITCH, LDS_RED = MT * NT * WAVES * 64 * 4;
   constexpr int LDS_F = LDS_WIN > LDS_RED ? LDS_WIN : LDS_RED;
   constexpr int LDS_FLOATS = LDS_F * 4 > JOB_LDS_BYTES ? LDS_F : JOB_LDS_BYTES / 4;
   __shared__ __attribute__((aligned(16))) float smem[LDS_FLOATS];
@@ -190,17 +311,25 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
   const int tile = (int)blockIdx.x - (has_job ? 1 : 0);
   const int xt = tile % a.rtiles;
   float* sF = smem;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, lane = tid & 63;
+  // the wave index is uniform: everything derived from it (the block schedule,
+  // the X block addresses, the loop bounds) lives in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int g = lane >> 4, c = lane & 15;
   const int split = tile / a.rtiles;
   const int w = split % a.nwin;
   const int ss = split / a.nwin;
-  const i64 s_lo = (i64)ss * a.n_slow / a.slow_splits;
-  const i64 s_hi = (i64)(ss + 1) * a.n_slow / a.slow_splits;
+  // balanced split of the slow range: the first slow_rem splits take one more
+  const i64 s_lo = uniform_i64((i64)ss * a.slow_q + (ss < a.slow_rem ? ss : a.slow_rem));
+  const i64 s_hi = s_lo + a.slow_q + (ss < a.slow_rem ? 1 : 0);
   const int nkb = a.kw / KB;
   const i64 my_slow = (s_hi - s_lo > wave) ? (s_hi - s_lo - wave + WAVES - 1) / WAVES : 0;
-  const i64 T = my_slow * nkb;
+  const int T = __builtin_amdgcn_readfirstlane((int)(my_slow * nkb));  // blocks of this wave
   const i64 row_base = (i64)xt * ROWS;
+
+  // window loads first (see stage_window_load)
+  FastWindow<RT, NTH, KWMAX> fw;
+  stage_window_load<WV>(fw, a, w);
 
   i64 rowc[MT];
   i64 rowv = 0;  // case B: first row of this lane's MT-vector
@@ -236,28 +365,24 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
   } else {
     loff[0] = (i64)(4 * g) * a.I + rowv;
   }
+  // PRE column of each lane: columns past rcount read column r0 (those
+  // accumulator columns are never stored)
+  const bool has_pre = a.pre != nullptr;
+  const float* pre_base = has_pre ? a.pre : kPreOnes;
+  const i64 pre_stride = has_pre ? a.R : 0;
   int poff[NT];
-  bool pok[NT];
 #pragma unroll
-  for (int nt = 0; nt < NT; ++nt) {
-    poff[nt] = a.r0 + 16 * nt + c;
-    pok[nt] = a.pre != nullptr && 16 * nt + c < a.rcount;
-  }
-  const i64 IQ = a.I * a.Q;
+  for (int nt = 0; nt < NT; ++nt) poff[nt] = (has_pre ? a.r0 : 0) + (16 * nt + c < a.rcount ? 16 * nt + c : 0);
 
-  auto load = [&](i64 t, Buf& b) {
-    if (t >= T) return;
-    const int tq = (int)t / nkb;
-    const int kb = (int)t - tq * nkb;
-    const i64 slow = s_lo + wave + (i64)tq * WAVES;
-    b.kb = kb;
-    const float* prow = a.pre + slow * a.R;
+  BlockCursor<WAVES, KB, CASE_B> cur(a, s_lo + wave, w, nkb);
+
+  auto load = [&](int t, Buf& b) {
+    b.kb = cur.kb;
+    const float* prow = pre_base + cur.slow * pre_stride;
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) b.pre[nt] = pok[nt] ? prow[poff[nt]] : 1.0f;
+    for (int nt = 0; nt < NT; ++nt) b.pre[nt] = prow[poff[nt]];
+    const float* ubase = X + cur.offset(a);
     if constexpr (!CASE_B) {
-      const int sl32 = (int)slow, qs32 = (int)a.qs;
-      const int p = sl32 / qs32;
-      const float* ubase = X + (i64)p * IQ + (i64)(sl32 - p * qs32) * a.fast_size + (i64)w * a.kw + kb * KB;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         const float* base = ubase + loff[mt];
@@ -265,8 +390,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
         b.x[mt][1] = *reinterpret_cast<const floatx4*>(base + 16);
       }
     } else {
-      const i64 j0 = slow * a.fast_size + (i64)w * a.kw + kb * KB;
-      const float* base = X + j0 * a.I + loff[0];
+      const float* base = ubase + loff[0];
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -285,9 +409,10 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
           }
         }
     }
+    if (t + 1 < T) cur.step(a);
   };
 
-  auto compute = [&](i64 t, const Buf& b) {
+  auto compute = [&](int t, const Buf& b) {
     if (t >= T) return;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -310,9 +435,9 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
   load(0, b0);
   load(1, b1);
   load(2, b2);
-  stage_fast_window<RT, PITCH, NTH, KWMAX>(sF, a, w);
+  stage_window_store<RT, PITCH, NTH, KWMAX>(sF, fw);
   __syncthreads();  // LDS window staged
-  for (i64 t = 0; t < T; t += 4) {
+  for (int t = 0; t < T; t += 4) {
     load(t + 3, b3);
     compute(t, b0);
     load(t + 4, b0);
@@ -323,22 +448,8 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
     compute(t + 3, b3);
   }
 
-  wg_reduce_acc<MT, NT, WAVES>(acc, smem, wave, lane);
-  if (wave != 0) return;
-  // D layout (16x16): lane (g, c), register v -> tile row 4g + v, column c
-  float* out = partial + (i64)split * a.I * a.R;
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      if (16 * nt + c >= a.rcount) continue;
-      const int col = a.r0 + 16 * nt + c;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const i64 row = CASE_B ? row_base + MT * (4 * g + v) + mt : row_base + 16 * mt + 4 * g + v;
-        if (row < a.I) out[row * a.R + col] = acc[mt][nt][v];
-      }
-    }
+  wg_reduce_store<MT, NT, WAVES>(acc, smem, wave, lane, partial + (i64)split * a.I * a.R, a.I, a.R, a.r0, a.rcount,
+                                 row_base, CASE_B);
 }
 
 // ---------------------------------------------------------------------------
@@ -349,11 +460,11 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
 // bf16 hi + lo pair (two MFMAs per step), so the input tensor is the only
 // operand rounded to bf16.  Accumulation is FP32.
 // ---------------------------------------------------------------------------
-template <int NT, int WAVES, int MT, int KS, int KWMAX>
+template <int NT, int WAVES, int MT, int KS, int KWMAX, bool WV>
 __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* __restrict__ X, KernArgs a,
                                                                float* __restrict__ partial, AlsJob job, int has_job) {
   constexpr int KB = 32 * KS, PITCH = KWMAX + 8, RT = 16 * NT, NTH = WAVES * 64, ROWS = 16 * MT;
-  constexpr int LDS_WIN_U16 = 2 * RT * PITCH, LDS_RED_F = (WAVES - 1) * MT * NT * 4 * 64;
+  constexpr int LDS_WIN_U16 = 2 * RT * PITCH, LDS_RED_F = MT * NT * WAVES * 64 * 4;
   constexpr int LDS_B = (LDS_WIN_U16 * 2 > LDS_RED_F * 4) ? LDS_WIN_U16 * 2 : LDS_RED_F * 4;
   constexpr int LDS_BYTES = LDS_B > JOB_LDS_BYTES ? LDS_B : JOB_LDS_BYTES;
   __shared__ __attribute__((aligned(16))) float smem[LDS_BYTES / 4];
@@ -366,17 +477,22 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
   const int xt = tile % a.rtiles;
   uint16_t* sHi = reinterpret_cast<uint16_t*>(smem);
   uint16_t* sLo = sHi + RT * PITCH;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar schedule and addresses
   const int g = lane >> 4, c = lane & 15;
   const int split = tile / a.rtiles;
   const int w = split % a.nwin;
   const int ss = split / a.nwin;
-  const i64 s_lo = (i64)ss * a.n_slow / a.slow_splits;
-  const i64 s_hi = (i64)(ss + 1) * a.n_slow / a.slow_splits;
+  // balanced split of the slow range: the first slow_rem splits take one more
+  const i64 s_lo = uniform_i64((i64)ss * a.slow_q + (ss < a.slow_rem ? ss : a.slow_rem));
+  const i64 s_hi = s_lo + a.slow_q + (ss < a.slow_rem ? 1 : 0);
   const int nkb = a.kw / KB;
   const i64 my_slow = (s_hi - s_lo > wave) ? (s_hi - s_lo - wave + WAVES - 1) / WAVES : 0;
-  const i64 T = my_slow * nkb;
+  const int T = __builtin_amdgcn_readfirstlane((int)(my_slow * nkb));  // blocks of this wave
   const i64 row_base = (i64)xt * ROWS;
+
+  FastWindow<RT, NTH, KWMAX> fw;  // window loads before the X blocks
+  stage_window_load<WV>(fw, a, w);
 
   i64 rowc[MT];
 #pragma unroll
@@ -404,36 +520,33 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
   i64 loff[MT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) loff[mt] = rowc[mt] * a.Q + 8 * g;
+  // PRE column of each lane: columns past rcount read column r0 (those
+  // accumulator columns are never stored)
+  const bool has_pre = a.pre != nullptr;
+  const float* pre_base = has_pre ? a.pre : kPreOnes;
+  const i64 pre_stride = has_pre ? a.R : 0;
   int poff[NT];
-  bool pok[NT];
 #pragma unroll
-  for (int nt = 0; nt < NT; ++nt) {
-    poff[nt] = a.r0 + 16 * nt + c;
-    pok[nt] = a.pre != nullptr && 16 * nt + c < a.rcount;
-  }
-  const i64 IQ = a.I * a.Q;
+  for (int nt = 0; nt < NT; ++nt) poff[nt] = (has_pre ? a.r0 : 0) + (16 * nt + c < a.rcount ? 16 * nt + c : 0);
 
-  auto load = [&](i64 t, Buf& b) {
-    if (t >= T) return;
-    const int tq = (int)t / nkb;
-    const int kb = (int)t - tq * nkb;
-    const i64 slow = s_lo + wave + (i64)tq * WAVES;
-    b.kb = kb;
-    const float* prow = a.pre + slow * a.R;
+  BlockCursor<WAVES, KB, false> cur(a, s_lo + wave, w, nkb);
+
+  auto load = [&](int t, Buf& b) {
+    b.kb = cur.kb;
+    const float* prow = pre_base + cur.slow * pre_stride;
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) b.pre[nt] = pok[nt] ? prow[poff[nt]] : 1.0f;
-    const int sl32 = (int)slow, qs32 = (int)a.qs;
-    const int p = sl32 / qs32;
-    const uint16_t* ubase = X + (i64)p * IQ + (i64)(sl32 - p * qs32) * a.fast_size + (i64)w * a.kw + kb * KB;
+    for (int nt = 0; nt < NT; ++nt) b.pre[nt] = prow[poff[nt]];
+    const uint16_t* ubase = X + cur.offset(a);
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
       const uint16_t* base = ubase + loff[mt];
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) b.x[mt][ks] = *reinterpret_cast<const bf16x8*>(base + 32 * ks);
     }
+    if (t + 1 < T) cur.step(a);
   };
 
-  auto compute = [&](i64 t, const Buf& b) {
+  auto compute = [&](int t, const Buf& b) {
     if (t >= T) return;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks)
@@ -455,9 +568,9 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
   load(0, b0);
   load(1, b1);
   load(2, b2);
-  stage_fast_window_bf16x2<RT, PITCH, NTH, KWMAX>(sHi, sLo, a, w);
+  stage_window_store_bf16x2<RT, PITCH, NTH, KWMAX>(sHi, sLo, fw);
   __syncthreads();  // LDS window staged
-  for (i64 t = 0; t < T; t += 4) {
+  for (int t = 0; t < T; t += 4) {
     load(t + 3, b3);
     compute(t, b0);
     load(t + 4, b0);
@@ -468,21 +581,8 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
     compute(t + 3, b3);
   }
 
-  wg_reduce_acc<MT, NT, WAVES>(acc, smem, wave, lane);
-  if (wave != 0) return;
-  float* out = partial + (i64)split * a.I * a.R;
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      if (16 * nt + c >= a.rcount) continue;
-      const int col = a.r0 + 16 * nt + c;
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const i64 row = row_base + 16 * mt + 4 * g + v;
-        if (row < a.I) out[row * a.R + col] = acc[mt][nt][v];
-      }
-    }
+  wg_reduce_store<MT, NT, WAVES>(acc, smem, wave, lane, partial + (i64)split * a.I * a.R, a.I, a.R, a.r0, a.rcount,
+                                 row_base, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -782,31 +882,36 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     a.n_slow = pl.J / pl.fast_size;
     a.qs = caseB ? 1 : pl.Q / pl.fast_size;
     a.rtiles = (int)rtiles;
+    a.win_vec = (R % 4 == 0) && ((uintptr_t)fastf % 16 == 0);
     // the kernel instance for this shape: calls f(kernel) with its function pointer
     auto with_kernel = [&](auto&& f) {
-      if (!bf16) {
-        if (!caseB) {
-          if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, false, KWMAX>);
-          if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 2, false, KWMAX>);
-          return f(mttkrp_f32_mfma<4, WAVES, 2, false, KWMAX>);
+      auto pick = [&](auto wv) {
+        constexpr bool WV = decltype(wv)::value;
+        if (!bf16) {
+          if (!caseB) {
+            if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, false, KWMAX, WV>);
+            if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 2, false, KWMAX, WV>);
+            return f(mttkrp_f32_mfma<4, WAVES, 2, false, KWMAX, WV>);
+          }
+          if (MT == 2) {
+            if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, true, KWMAX, WV>);
+            if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 2, true, KWMAX, WV>);
+            return f(mttkrp_f32_mfma<4, WAVES, 2, true, KWMAX, WV>);
+          }
+          if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 1, true, KWMAX, WV>);
+          if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 1, true, KWMAX, WV>);
+          return f(mttkrp_f32_mfma<4, WAVES, 1, true, KWMAX, WV>);
         }
-        if (MT == 2) {
-          if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, true, KWMAX>);
-          if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 2, true, KWMAX>);
-          return f(mttkrp_f32_mfma<4, WAVES, 2, true, KWMAX>);
+        if (KB == 64) {
+          if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES, 2, 2, KWMAX, WV>);
+          if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES, 2, 2, KWMAX, WV>);
+          return f(mttkrp_bf16_mfma<4, WAVES, 2, 2, KWMAX, WV>);
         }
-        if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 1, true, KWMAX>);
-        if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 1, true, KWMAX>);
-        return f(mttkrp_f32_mfma<4, WAVES, 1, true, KWMAX>);
-      }
-      if (KB == 64) {
-        if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES, 2, 2, KWMAX>);
-        if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES, 2, 2, KWMAX>);
-        return f(mttkrp_bf16_mfma<4, WAVES, 2, 2, KWMAX>);
-      }
-      if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES, 2, 1, KWMAX>);
-      if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES, 2, 1, KWMAX>);
-      return f(mttkrp_bf16_mfma<4, WAVES, 2, 1, KWMAX>);
+        if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES, 2, 1, KWMAX, WV>);
+        if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES, 2, 1, KWMAX, WV>);
+        return f(mttkrp_bf16_mfma<4, WAVES, 2, 1, KWMAX, WV>);
+      };
+      return a.win_vec ? pick(std::true_type{}) : pick(std::false_type{});
     };
     // workgroups per CU doing MTTKRP work: 2 (the sweet spot swept at 256^3 and
     // 512x512x128), fewer if the instance cannot keep 2 resident, or the
@@ -843,6 +948,8 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     ssp = std::min<i64>(ssp, a.n_slow >= WAVES ? a.n_slow / WAVES : 1);
     ssp = std::min<i64>(ssp, 65535 / a.nwin);
     a.slow_splits = (int)std::max<i64>(1, ssp);
+    a.slow_q = a.n_slow / a.slow_splits;
+    a.slow_rem = a.n_slow % a.slow_splits;
     splits = a.nwin * a.slow_splits;
     void* pbuf = nullptr;
     TENRSO_TRY(ctx_buffer(ctx, SLOT_PARTIAL, (size_t)splits * nout * sizeof(float), &pbuf));
