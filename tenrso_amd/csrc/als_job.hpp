@@ -277,10 +277,6 @@ __device__ __forceinline__ bool job_block_gj(doublex4 (&acc)[4], double* lds) {
 __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
   const int R = a.R;
   // entries of this thread: (job_row(e), job_col(t)), t, e = 0..3
-  // Inside an MTTKRP launch the job's waves share their SIMDs with MTTKRP
-  // waves; the job is a chain of short dependent steps, so it takes issue
-  // priority and the MTTKRP waves fill its latency gaps.
-  __builtin_amdgcn_s_setprio(3);
   TENRSO_JOB_MARK(0);
   const bool want_fit = a.do_fit != 0;
   // <X, Xhat> partials: loads issued ahead of the Gram loads
@@ -297,9 +293,9 @@ __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
       cross[t][e] = 1.0;
       gam[t][e] = 1.0;
     }
-  // Gram blocks of modes k0, k0 + 1 loaded together (two round trips fewer for
-  // three modes than one mode at a time; the MTTKRP beside the job keeps the
-  // memory system busy, so each round trip is long)
+  // Gram blocks of up to four modes loaded together: one round trip for a
+  // tensor of order <= 4 (the MTTKRP beside the job keeps the memory system
+  // busy, so each round trip is long)
   auto load_block = [&](int k, double (&g)[4][4]) {
     const double* Gk = a.G + (i64)k * R * R;
 #pragma unroll
@@ -321,12 +317,16 @@ __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
         gam[t][e] = in_gam ? gam[t][e] * g[t][e] : gam[t][e];
       }
   };
-  for (int k = 0; k < a.nd; k += 2) {
-    double g0[4][4], g1[4][4];
+  for (int k = 0; k < a.nd; k += 4) {
+    double g0[4][4], g1[4][4], g2[4][4], g3[4][4];
     load_block(k, g0);
     load_block(k + 1, g1);
+    load_block(k + 2, g2);
+    load_block(k + 3, g3);
     fold(k, g0);
     fold(k + 1, g1);
+    fold(k + 2, g2);
+    fold(k + 3, g3);
   }
   if (want_fit) {
     // ||Xhat||^2 = sum_{r,s} prod_k G_k[r,s]; <X, Xhat> = sum of the partials
@@ -338,22 +338,30 @@ __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
     const double recon = job_block_sum(rec, lds);
     const double ip = job_block_sum(ipv, lds + 8);
     if (threadIdx.x == 0) {
-      const double nx = a.scal[JOB_SC_NORM_X];
+      // the scalar block as it stands (the solve status of the sweep just
+      // evaluated included), read in one round trip before any store to it
+      double blk[JOB_SC_STAMP];
+#pragma unroll
+      for (int i = 0; i < JOB_SC_STAMP; ++i) blk[i] = a.scal[i];
+      const double nx = blk[JOB_SC_NORM_X];
       const double err_sq = nx + recon - 2.0 * ip;
       const double err = sqrt(err_sq > 0.0 ? err_sq : 0.0);
       double fit = 1.0 - err / sqrt(nx);
       fit = fit < 0.0 ? 0.0 : (fit > 1.0 ? 1.0 : fit);
+      blk[JOB_SC_FIT] = fit;
+      blk[JOB_SC_IP] = ip;
+      blk[JOB_SC_RECON] = recon;
       a.scal[JOB_SC_FIT] = fit;
       a.scal[JOB_SC_IP] = ip;
       a.scal[JOB_SC_RECON] = recon;
       if (a.host) {
-        // the scalar block as it stands (the solve status of the sweep just
-        // evaluated included) straight into host memory, then the stamp the
-        // host polls for: no copy launch, no event
-        for (int i = 0; i < JOB_SC_STAMP; ++i) a.host[i] = a.scal[i];
+        // straight into host memory, then the stamp the host polls for (no copy
+        // launch, no event); the fence orders the block before the stamp, and
+        // nothing here waits for the stamp itself to land
+#pragma unroll
+        for (int i = 0; i < JOB_SC_STAMP; ++i) a.host[i] = blk[i];
         __threadfence_system();
         a.host[JOB_SC_STAMP] = a.stamp;
-        __threadfence_system();
       }
     }
     __syncthreads();
@@ -373,14 +381,26 @@ __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
   // back from Winv, so gam is dead once the first attempt's matrix is formed
   auto form = [&](bool retry) {
     const double ridge = retry ? (double)FLT_EPSILON * (double)(10 * R) : 0.0;
+    // uniform branch: without any regularisation term the band arithmetic of
+    // job_reg is not executed at all
+    if (retry || a.tik_order > 0 || a.l2 > 0.0) {
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+      for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int i = job_row(e), col = job_col(t);
-        const double g = retry ? a.Winv[i * JOB_NE + col] : gam[t][e];
-        acc[t][e] = (i < R && col < R) ? g + job_reg(a, i, col, ridge) : (i == col ? 1.0 : 0.0);
-      }
+        for (int e = 0; e < 4; ++e) {
+          const int i = job_row(e), col = job_col(t);
+          const double g = retry ? a.Winv[i * JOB_NE + col] : gam[t][e];
+          acc[t][e] = (i < R && col < R) ? g + job_reg(a, i, col, ridge) : (i == col ? 1.0 : 0.0);
+        }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int i = job_row(e), col = job_col(t);
+          acc[t][e] = (i < R && col < R) ? gam[t][e] : (i == col ? 1.0 : 0.0);
+        }
+    }
   };
   form(false);
 #pragma unroll 1

@@ -162,6 +162,12 @@ __global__ __launch_bounds__(256) void solve_rows_kernel(const float* __restrict
   const int c = threadIdx.x & 63, qd = threadIdx.x >> 6;
   const i64 row = blockIdx.x;
   const bool colok = c < R;
+  // this thread's quarter of Winv column c, loaded beside the partials (it does
+  // not depend on them): one memory round trip instead of two
+  const int r0 = qd * R / 4, r1 = (qd + 1) * R / 4;
+  double w[NE / 4];
+#pragma unroll
+  for (int k = 0; k < NE / 4; ++k) w[k] = (colok && r0 + k < r1) ? Winv[(r0 + k) * NE + c] : 0.0;
   double m = 0.0;
   if (colok) {
     if (part) {
@@ -185,17 +191,16 @@ __global__ __launch_bounds__(256) void solve_rows_kernel(const float* __restrict
   __syncthreads();
   if (qd == 0) sM[c] = (sq[c] + sq[NE + c]) + (sq[2 * NE + c] + sq[3 * NE + c]);
   __syncthreads();
-  // x[c] = sum_r M[r] Winv[r, c], r split over the four quarters
+  // x[c] = sum_r M[r] Winv[r, c], r split over the four quarters (two
+  // alternating sums per thread, as before the prefetch)
   double x = 0.0;
   if (colok) {
-    const int r0 = qd * R / 4, r1 = (qd + 1) * R / 4;
     double x0 = 0, x1 = 0;
-    int r = r0;
-    for (; r + 2 <= r1; r += 2) {
-      x0 += sM[r] * Winv[r * NE + c];
-      x1 += sM[r + 1] * Winv[(r + 1) * NE + c];
+#pragma unroll
+    for (int k = 0; k < NE / 4; k += 2) {
+      if (r0 + k < r1) x0 += sM[r0 + k] * w[k];
+      if (r0 + k + 1 < r1) x1 += sM[r0 + k + 1] * w[k + 1];
     }
-    if (r < r1) x0 += sM[r] * Winv[r * NE + c];
     x = x0 + x1;
   }
   sq[qd * NE + c] = x;

@@ -81,8 +81,9 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
 template <int RT, int NTH, int KWMAX>
 struct FastWindow {
   static constexpr int UNITS = RT / 4 * (KWMAX / 2);
-  static constexpr int PERU = UNITS / NTH;
-  static_assert(UNITS % NTH == 0, "window units must split evenly over the workgroup");
+  static constexpr int PERU = (UNITS + NTH - 1) / NTH;
+  // every unit index below PERU*NTH maps in bounds: u >= UNITS gives kk >= KWMAX
+  static __device__ __forceinline__ bool live(int u) { return UNITS % NTH == 0 || u < UNITS; }
   floatx4 v[PERU][2];
 };
 
@@ -123,6 +124,7 @@ __device__ __forceinline__ void stage_window_store(float* __restrict__ s, const 
   for (int e = 0; e < FastWindow<RT, NTH, KWMAX>::PERU; ++e) {
     const int u = threadIdx.x + e * NTH;
     const int r4 = 4 * (u % (RT / 4)), kk = 2 * (u / (RT / 4));
+    if (!FastWindow<RT, NTH, KWMAX>::live(u)) continue;
 #pragma unroll
     for (int q = 0; q < 4; ++q)
       *reinterpret_cast<float2*>(&s[(r4 + q) * PITCH + kk]) = float2{fw.v[e][0][q], fw.v[e][1][q]};
@@ -139,6 +141,7 @@ __device__ __forceinline__ void stage_window_store_bf16x2(uint16_t* __restrict__
   for (int e = 0; e < FastWindow<RT, NTH, KWMAX>::PERU; ++e) {
     const int u = threadIdx.x + e * NTH;
     const int r4 = 4 * (u % (RT / 4)), kk = 2 * (u / (RT / 4));
+    if (!FastWindow<RT, NTH, KWMAX>::live(u)) continue;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       // v_cvt_pk_bf16_f32 rounds to nearest even, as f32_to_bf16_rne does for
@@ -169,8 +172,8 @@ __device__ float kPreOnes[64] = {TENRSO_ONES8, TENRSO_ONES8, TENRSO_ONES8, TENRS
 
 // Block schedule of one wave.  Block t of the wave is block kb = t % nkb of the
 // slow index s_lo + wave + WAVES*(t / nkb).  Loads run in t order, so the cursor
-// only steps forward, and it stops on the wave's last block: loads past the end
-// re-read that block (a cache hit).  Every X and PRE load is thus unconditional
+// only steps forward, and it stops on the wave's last block (loads past the end
+// are redirected by the kernels to one cache line).  Every X and PRE load is thus unconditional
 // and the wait before a block's MFMAs counts exactly the loads issued after it,
 // which keeps the prefetch in flight.  A wave with no block reads the block of
 // slow index min(s_lo + wave, n_slow - 1), always inside X, and computes nothing.
@@ -381,16 +384,20 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
     const float* prow = pre_base + cur.slow * pre_stride;
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) b.pre[nt] = prow[poff[nt]];
-    const float* ubase = X + cur.offset(a);
+    // past the last block every lane reads the first elements of X (one cache
+    // line, inside X for every shape this kernel takes): the load stays
+    // unconditional and costs the memory pipeline almost nothing
+    const bool live = t < T;
+    const float* ubase = X + (live ? cur.offset(a) : 0);
     if constexpr (!CASE_B) {
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        const float* base = ubase + loff[mt];
+        const float* base = ubase + (live ? loff[mt] : 0);
         b.x[mt][0] = *reinterpret_cast<const floatx4*>(base);
         b.x[mt][1] = *reinterpret_cast<const floatx4*>(base + 16);
       }
     } else {
-      const float* base = ubase + loff[0];
+      const float* base = ubase + (live ? loff[0] : 0);
 #pragma unroll
       for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -468,9 +475,12 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
   constexpr int LDS_B = (LDS_WIN_U16 * 2 > LDS_RED_F * 4) ? LDS_WIN_U16 * 2 : LDS_RED_F * 4;
   constexpr int LDS_BYTES = LDS_B > JOB_LDS_BYTES ? LDS_B : JOB_LDS_BYTES;
   __shared__ __attribute__((aligned(16))) float smem[LDS_BYTES / 4];
-  static_assert(NTH == JOB_THREADS, "the ALS job needs a 256-thread workgroup");
+  static_assert(NTH % JOB_THREADS == 0, "the ALS job runs on the first 256 threads");
   if (has_job && blockIdx.x == 0) {
-    als_job_run(job, reinterpret_cast<double*>(smem));
+    // the job's workgroup: waves past the job's four end at once (a barrier
+    // no longer waits for an ended wave); with one such workgroup per CU the
+    // job has its CU to itself
+    if (threadIdx.x < JOB_THREADS) als_job_run(job, reinterpret_cast<double*>(smem));
     return;
   }
   const int tile = (int)blockIdx.x - (has_job ? 1 : 0);
@@ -536,10 +546,11 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
     const float* prow = pre_base + cur.slow * pre_stride;
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) b.pre[nt] = prow[poff[nt]];
-    const uint16_t* ubase = X + cur.offset(a);
+    const bool live = t < T;  // past the end: one cache line at X (see the FP32 kernel)
+    const uint16_t* ubase = X + (live ? cur.offset(a) : 0);
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
-      const uint16_t* base = ubase + loff[mt];
+      const uint16_t* base = ubase + (live ? loff[mt] : 0);
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) b.x[mt][ks] = *reinterpret_cast<const bf16x8*>(base + 32 * ks);
     }
@@ -872,7 +883,12 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
   float* partial = nullptr;
   int splits = 1;
   if (mfma_ok) {
-    constexpr int WAVES = 4;
+    // FP32: 4-wave workgroups, two per CU.  BF16: 8-wave workgroups, one per CU
+    // (the VGPR budget admits one), so a fused ALS job gets a CU of its own
+    // instead of sharing one with an MTTKRP workgroup (measured: the job at
+    // ~22 us beside the MTTKRP against ~14 us alone).
+    constexpr int WAVES = 4, WAVES_BF16 = 8;
+    const int waves = bf16 ? WAVES_BF16 : WAVES;
     const int MT = (caseB && (I % 2 != 0)) ? 1 : 2;
     const i64 rows_per_wg = 16 * MT;
     const i64 rtiles = (i64)cdiv(I, rows_per_wg);
@@ -903,21 +919,21 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
           return f(mttkrp_f32_mfma<4, WAVES, 1, true, KWMAX, WV>);
         }
         if (KB == 64) {
-          if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES, 2, 2, KWMAX, WV>);
-          if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES, 2, 2, KWMAX, WV>);
-          return f(mttkrp_bf16_mfma<4, WAVES, 2, 2, KWMAX, WV>);
+          if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES_BF16, 2, 2, KWMAX, WV>);
+          if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES_BF16, 2, 2, KWMAX, WV>);
+          return f(mttkrp_bf16_mfma<4, WAVES_BF16, 2, 2, KWMAX, WV>);
         }
-        if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES, 2, 1, KWMAX, WV>);
-        if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES, 2, 1, KWMAX, WV>);
-        return f(mttkrp_bf16_mfma<4, WAVES, 2, 1, KWMAX, WV>);
+        if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES_BF16, 2, 1, KWMAX, WV>);
+        if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES_BF16, 2, 1, KWMAX, WV>);
+        return f(mttkrp_bf16_mfma<4, WAVES_BF16, 2, 1, KWMAX, WV>);
       };
       return a.win_vec ? pick(std::true_type{}) : pick(std::false_type{});
     };
-    // workgroups per CU doing MTTKRP work: 2 (the sweet spot swept at 256^3 and
-    // 512x512x128), fewer if the instance cannot keep 2 resident, or the
-    // TENRSO_WG_PER_CU_* override.  When the ALS job shares the launch it takes a
-    // third resident slot on one CU if the instance fits 3, otherwise one of the
-    // MTTKRP slots -- either way every workgroup is resident at once.
+    // workgroups per CU doing MTTKRP work: 2 four-wave FP32 workgroups (the sweet
+    // spot swept at 256^3 and 512x512x128), one eight-wave BF16 workgroup, fewer if
+    // the instance cannot keep that many resident, or the TENRSO_WG_PER_CU_*
+    // override.  When the ALS job shares the launch it takes one of the resident
+    // slots, so every workgroup is resident at once.
     int wpc = bf16 ? ctx->wg_per_cu_bf16 : ctx->wg_per_cu_f32;
     int occ = 0;
     {
@@ -931,7 +947,7 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
       if (occ <= 0) {
         int nb = 0;
         TENRSO_HIP_TRY(with_kernel([&](auto kern) {
-          return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, WAVES * 64, 0);
+          return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, waves * 64, 0);
         }));
         occ = std::max(1, nb);
         if (ctx->occ_n < 32) {
@@ -945,7 +961,7 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     const i64 target = (i64)wpc * ctx->num_cus - ((fuse_job && occ <= wpc) ? 1 : 0);
     // each workgroup keeps >= WAVES slow indices
     i64 ssp = std::max<i64>(1, target / std::max<i64>(1, tiles) / a.nwin);
-    ssp = std::min<i64>(ssp, a.n_slow >= WAVES ? a.n_slow / WAVES : 1);
+    ssp = std::min<i64>(ssp, a.n_slow >= waves ? a.n_slow / waves : 1);
     ssp = std::min<i64>(ssp, 65535 / a.nwin);
     a.slow_splits = (int)std::max<i64>(1, ssp);
     a.slow_q = a.n_slow / a.slow_splits;
@@ -955,7 +971,7 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     TENRSO_TRY(ctx_buffer(ctx, SLOT_PARTIAL, (size_t)splits * nout * sizeof(float), &pbuf));
     partial = (float*)pbuf;
     if (job && !fuse_job) TENRSO_TRY(als_job_launch(ctx, *job));
-    const dim3 block(WAVES * 64);
+    const dim3 block(waves * 64);
     auto launch_main = [&](bool with_job) -> int {
       for (int rc = 0; rc < rchunks; ++rc) {
         const int has_job = (with_job && fuse_job && rc == 0) ? 1 : 0;
