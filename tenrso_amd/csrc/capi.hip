@@ -428,7 +428,7 @@ int tenrso_sp_cp_als_f32(tenrso_ctx* ctx, const tenrso_sptensor* t, float* const
   AlsProblem pb;
   pb.nd = t->nd;
   pb.shape = t->shape;
-  pb.norm_min = [&](double* tmp, double* scal) { return tensor_norm_min(ctx, t->vals, t->nnz, tmp, scal); };
+  pb.norm_min = [&](double* tmp, double* scal) { return tensor_norm_min(ctx, t->vals, t->nnz, tmp, scal, SC_COUNT); };
   pb.mttkrp = [&](int mode, const float* const* F, const AlsJob* job, MttkrpResult* mr) -> int {
     if (job) TENRSO_TRY(als_job_launch(ctx, *job));
     i64 maxI = 0;

@@ -105,8 +105,7 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
     }
   }
 
-  TENRSO_TRY(fill_f64(ctx, scal, SC_COUNT, 0.0));
-  TENRSO_TRY(pb.norm_min(ip_part, scal));
+  TENRSO_TRY(pb.norm_min(ip_part, scal));  // also clears the rest of the scalar block
 
   const bool nonneg = o->nonnegative != 0;
   double* pinned = (double*)ctx->pinned;  // [2][SC_COUNT]
@@ -122,7 +121,7 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
     for (int k = 0; k < nd; ++k) TENRSO_TRY(clamp_nonneg(ctx, F[k], shape[k] * R));  // core.rs:281-285
   if (pb.prepare) TENRSO_TRY(pb.prepare());
 
-  for (int k = 0; k < nd; ++k) TENRSO_TRY(gram_f64(ctx, F[k], shape[k], R, G + (i64)k * R * R));
+  TENRSO_TRY(gram_f64_many(ctx, nd, (const float* const*)F, shape, R, G));
 
   RowEpilogue epi;
   epi.l1 = o->l1;
@@ -315,12 +314,33 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
   AlsProblem pb;
   pb.nd = nd;
   pb.shape = shape;
-  pb.norm_min = [&](double* tmp, double* scal) { return tensor_norm_min(ctx, X, total, tmp, scal); };
+  // BF16: the two bf16 copies of X and its norm come from one pass over X when
+  // the shape allows it (prepare_bf16_als); otherwise norm, copy and transposed
+  // copy are three passes
+  bool prepared = false;
+  auto bf16_buffers = [&](void** pb16, void** pt) -> int {
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16, (size_t)total * 2, pb16));
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * 2, pt));
+    return TENRSO_OK;
+  };
+  pb.norm_min = [&](double* tmp, double* scal) -> int {
+    if (bf16) {
+      void *pb16, *pt;
+      TENRSO_TRY(bf16_buffers(&pb16, &pt));
+      int nblk = 0;
+      if (prepare_bf16_als(ctx, X, nd, shape, (uint16_t*)pb16, (uint16_t*)pt, tmp, &nblk) == TENRSO_OK) {
+        Xk = pb16;
+        Xt = pt;
+        prepared = true;
+        return norm_min_finish(ctx, tmp, nblk, scal, SC_COUNT);
+      }
+    }
+    return tensor_norm_min(ctx, X, total, tmp, scal, SC_COUNT);
+  };
   pb.prepare = [&]() -> int {
-    if (!bf16) return TENRSO_OK;
+    if (!bf16 || prepared) return TENRSO_OK;
     void *pb16, *pt;
-    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16, (size_t)total * 2, &pb16));
-    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * 2, &pt));
+    TENRSO_TRY(bf16_buffers(&pb16, &pt));
     TENRSO_TRY(convert_bf16(ctx, X, nd, shape, false, (uint16_t*)pb16));
     TENRSO_TRY(convert_bf16(ctx, X, nd, shape, true, (uint16_t*)pt));
     Xk = pb16;

@@ -17,6 +17,8 @@ struct AlsJob;
 struct AlsProblem {
   int nd = 0;
   const i64* shape = nullptr;
+  // ||X||^2 and min(X) into scal[SC_NORM_X], scal[SC_MIN_X], the other slots of
+  // the scalar block cleared; tmp: scratch for the partials
   std::function<int(double* tmp, double* scal)> norm_min;
   std::function<int()> prepare;
   std::function<int(int mode, const float* const* F, const AlsJob* job, MttkrpResult* out)> mttkrp;

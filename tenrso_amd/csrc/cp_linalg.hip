@@ -35,17 +35,47 @@ __device__ double block_sum_256(double v, double* red) {
 }
 
 // ---------------------------------------------------------------- ||X||^2, min X
+// Grid-stride over X with four independent 16-byte loads per thread and step
+// (VEC: X 16-byte aligned), FP64 sums of the exact fp32 squares; one (sum, min)
+// partial per workgroup, reduced by norm_min_final.
+template <bool VEC>
 __global__ __launch_bounds__(256) void norm_min_partial(const float* __restrict__ X, i64 n, double* __restrict__ part) {
   __shared__ double red[256];
-  double s = 0.0;
+  double s0 = 0.0, s1 = 0.0;
   float mn = FLT_MAX;
   const i64 stride = (i64)gridDim.x * 256;
-  for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
-    const float x = X[i];
-    s += (double)x * (double)x;
+  const i64 g = (i64)blockIdx.x * 256 + threadIdx.x;
+  auto add = [&](float x) {
+    s0 = fma((double)x, (double)x, s0);
     mn = fminf(mn, x);
+  };
+  if constexpr (VEC) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4* X4 = reinterpret_cast<const f4*>(X);
+    const i64 n4 = n / 4;
+    i64 i = g;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+      const f4 a = X4[i], b = X4[i + stride], c = X4[i + 2 * stride], d = X4[i + 3 * stride];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        add(a[q]);
+        s1 = fma((double)b[q], (double)b[q], s1);
+        mn = fminf(mn, b[q]);
+        add(c[q]);
+        s1 = fma((double)d[q], (double)d[q], s1);
+        mn = fminf(mn, d[q]);
+      }
+    }
+    for (; i < n4; i += stride) {
+      const f4 a = X4[i];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) add(a[q]);
+    }
+    for (i64 j = 4 * n4 + g; j < n; j += stride) add(X[j]);
+  } else {
+    for (i64 i = g; i < n; i += stride) add(X[i]);
   }
-  const double tot = block_sum_256(s, red);
+  const double tot = block_sum_256(s0 + s1, red);
   red[threadIdx.x] = (double)mn;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
@@ -58,13 +88,18 @@ __global__ __launch_bounds__(256) void norm_min_partial(const float* __restrict_
   }
 }
 
-__global__ __launch_bounds__(256) void norm_min_final(const double* __restrict__ part, int nblk, double* __restrict__ scal) {
+// Adds the partials; also clears scal[0, clear) first (the scalar block of a
+// new run), so no separate fill launch is needed.
+__global__ __launch_bounds__(256) void norm_min_final(const double* __restrict__ part, int nblk,
+                                                      double* __restrict__ scal, int clear) {
   __shared__ double red[256];
   double s = 0.0, mn = DBL_MAX;
   for (int b = threadIdx.x; b < nblk; b += 256) {
     s += part[b];
     mn = fmin(mn, part[nblk + b]);
   }
+  for (int i = threadIdx.x; i < clear; i += 256)
+    if (i != SC_NORM_X && i != SC_MIN_X) scal[i] = 0.0;
   const double tot = block_sum_256(s, red);
   red[threadIdx.x] = mn;
   __syncthreads();
@@ -78,20 +113,27 @@ __global__ __launch_bounds__(256) void norm_min_final(const double* __restrict__
   }
 }
 
-int tensor_norm_min(tenrso_ctx* ctx, const float* X, i64 n, double* tmp, double* scal) {
-  const int nblk = (int)std::min<i64>(1024, std::max<i64>(1, cdiv(n, 256 * 16)));
-  hipLaunchKernelGGL(norm_min_partial, dim3(nblk), dim3(256), 0, ctx->stream, X, n, tmp);
-  hipLaunchKernelGGL(norm_min_final, dim3(1), dim3(256), 0, ctx->stream, tmp, nblk, scal);
+int norm_min_finish(tenrso_ctx* ctx, const double* part, int nblk, double* scal, int clear) {
+  hipLaunchKernelGGL(norm_min_final, dim3(1), dim3(256), 0, ctx->stream, part, nblk, scal, clear);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
+}
+
+int tensor_norm_min(tenrso_ctx* ctx, const float* X, i64 n, double* tmp, double* scal, int clear) {
+  const int nblk = (int)std::min<i64>(1024, std::max<i64>(1, cdiv(n, 256 * 16)));
+  if ((uintptr_t)X % 16 == 0)
+    hipLaunchKernelGGL(norm_min_partial<true>, dim3(nblk), dim3(256), 0, ctx->stream, X, n, tmp);
+  else
+    hipLaunchKernelGGL(norm_min_partial<false>, dim3(nblk), dim3(256), 0, ctx->stream, X, n, tmp);
+  TENRSO_HIP_TRY(hipGetLastError());
+  return norm_min_finish(ctx, tmp, nblk, scal, clear);
 }
 
 // ---------------------------------------------------------------- Gram A^T A
 // grid: R workgroups of 1024 threads; workgroup r computes Gram row r.  Thread
 // (ch, c) sums A[i][r] A[i][c] in FP64 over row chunk ch (16 chunks), with all of
 // its loads independent; the 16 chunk sums are added in fixed order.
-__global__ __launch_bounds__(1024) void gram_rows_kernel(const float* __restrict__ A, i64 rows, int R,
-                                                         double* __restrict__ G) {
+__device__ __forceinline__ void gram_rows_body(const float* __restrict__ A, i64 rows, int R, double* __restrict__ G) {
   __shared__ double part[16 * SOLVE_RMAX];
   const int c = threadIdx.x & 63, ch = threadIdx.x >> 6;
   const int r = blockIdx.x;
@@ -113,6 +155,37 @@ __global__ __launch_bounds__(1024) void gram_rows_kernel(const float* __restrict
     for (int k = 0; k < 16; ++k) s += part[k * SOLVE_RMAX + c];
     G[(i64)r * R + c] = s;
   }
+}
+
+__global__ __launch_bounds__(1024) void gram_rows_kernel(const float* __restrict__ A, i64 rows, int R,
+                                                         double* __restrict__ G) {
+  gram_rows_body(A, rows, R, G);
+}
+
+// The Grams of several factors in one launch: grid (R, count), G_k at G + k*R*R.
+struct GramMany {
+  const float* A[MAXD];
+  i64 rows[MAXD];
+};
+__global__ __launch_bounds__(1024) void gram_many_kernel(GramMany m, int R, double* __restrict__ G) {
+  const int k = blockIdx.y;
+  gram_rows_body(m.A[k], m.rows[k], R, G + (i64)k * R * R);
+}
+
+int gram_f64_many(tenrso_ctx* ctx, int count, const float* const* A, const i64* rows, i64 R, double* G) {
+  if (R > SOLVE_RMAX) {
+    set_error("rank %lld exceeds the on-device Gram bound %d", (long long)R, SOLVE_RMAX);
+    return TENRSO_ERR_UNSUPPORTED;
+  }
+  if (R == 0 || count == 0) return TENRSO_OK;
+  GramMany m{};
+  for (int k = 0; k < count; ++k) {
+    m.A[k] = A[k];
+    m.rows[k] = rows[k];
+  }
+  hipLaunchKernelGGL(gram_many_kernel, dim3((unsigned)R, (unsigned)count), dim3(1024), 0, ctx->stream, m, (int)R, G);
+  TENRSO_HIP_TRY(hipGetLastError());
+  return TENRSO_OK;
 }
 
 int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G, hipStream_t stream) {

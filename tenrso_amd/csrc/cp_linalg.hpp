@@ -32,9 +32,13 @@ enum Scalars : int {
 };
 
 // ||X||^2 (FP64) and min(X) into scal[SC_NORM_X], scal[SC_MIN_X].  tmp: >= 2*1024 doubles.
-int tensor_norm_min(tenrso_ctx* ctx, const float* X, i64 n, double* tmp, double* scal);
+int tensor_norm_min(tenrso_ctx* ctx, const float* X, i64 n, double* tmp, double* scal, int clear = 0);
+// final reduction of (sum, min) partials [nblk sums, nblk minima] into scal; clears scal[0, clear) first
+int norm_min_finish(tenrso_ctx* ctx, const double* part, int nblk, double* scal, int clear);
 // G = A^T A (FP64) for an fp32 rows x R factor, on `stream` (null: the context stream).
 int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G, hipStream_t stream = nullptr);
+// Grams of count factors in one launch, G_k at G + k*R*R
+int gram_f64_many(tenrso_ctx* ctx, int count, const float* const* A, const i64* rows, i64 R, double* G);
 // A[i,:] = epi(M[i,:] Winv), M = sum of the `splits` fp32 partials [splits][rows][R]
 // (or the fp64 M when part is null); ip_part[blk] = sum_i M[i,:].A[i,:] (FP64).
 int solve_rows(tenrso_ctx* ctx, const float* part, int splits, const double* M, i64 rows, i64 R,

@@ -38,6 +38,12 @@ int reduce_partials(tenrso_ctx* ctx, const float* partial, int splits, i64 n, fl
 // fp32 -> bf16 (RNE) copy; when `last_to_front` the output is the tensor with its
 // last mode moved to the front ([I_{N-1}, I_0, ..., I_{N-2}]).
 int convert_bf16(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, bool last_to_front, uint16_t* dst);
+// BF16 ALS input in one pass: dst = bf16(X), dstT = its last-to-front copy, and
+// (sum of squares, min) partials in tmp[0, 2 * nblk).  TENRSO_ERR_UNSUPPORTED
+// (nothing launched) unless the last mode is a multiple of 4 and X is 16-byte
+// aligned.
+int prepare_bf16_als(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, uint16_t* dst, uint16_t* dstT,
+                     double* tmp, int* nblk_out);
 
 // bf16 [.., I_{N-1}] -> [I_{N-1}, ..] (last mode moved to the front), plain copy.
 int transpose_last_to_front_u16(tenrso_ctx* ctx, const uint16_t* src, int nd, const i64* shape, uint16_t* dst);

@@ -759,6 +759,104 @@ int convert_bf16(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, bo
   return TENRSO_OK;
 }
 
+// One pass over an fp32 tensor viewed as [rows, K] (K = last mode) for the BF16
+// ALS: the bf16 copy (same layout), the last-to-front bf16 copy ([K, rows]) and
+// per-workgroup (sum of squares, min) partials, instead of three passes.  64 x 64
+// tiles, grid-stride over tiles; each thread loads 4 x 16 bytes (K % 4 == 0, X
+// 16-byte aligned), writes its bf16 quads straight out and through an LDS tile
+// the transposed quads.  The bf16 values are f32_to_bf16_rne, as in the
+// separate conversion kernels.
+__global__ __launch_bounds__(256) void prepare_bf16_kernel(const float* __restrict__ X, i64 rows, i64 K,
+                                                           uint16_t* __restrict__ dst, uint16_t* __restrict__ dstT,
+                                                           double* __restrict__ part) {
+  __shared__ uint16_t tile[64][64 + 4];
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  const i64 tr_n = (rows + 63) / 64, tk_n = (K + 63) / 64;
+  double ss = 0.0;
+  float mn = FLT_MAX;
+  for (i64 t = blockIdx.x; t < tr_n * tk_n; t += gridDim.x) {
+    const i64 r0 = (t / tk_n) * 64, k0 = (t % tk_n) * 64;
+    // load + direct copy: thread -> (row tid / 16 + 16 j, columns 4 (tid % 16) ..)
+    const int c4 = 4 * (tid & 15);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int rr = (tid >> 4) + 16 * j;
+      const i64 r = r0 + rr, k = k0 + c4;
+      ushort4 o = {0, 0, 0, 0};
+      if (r < rows && k < K) {
+        const floatx4 v = *reinterpret_cast<const floatx4*>(X + r * K + k);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          ss = fma((double)v[q], (double)v[q], ss);
+          mn = fminf(mn, v[q]);
+        }
+        o.x = f32_to_bf16_rne(v[0]);
+        o.y = f32_to_bf16_rne(v[1]);
+        o.z = f32_to_bf16_rne(v[2]);
+        o.w = f32_to_bf16_rne(v[3]);
+        *reinterpret_cast<ushort4*>(dst + r * K + k) = o;
+      }
+      tile[rr][c4] = o.x;
+      tile[rr][c4 + 1] = o.y;
+      tile[rr][c4 + 2] = o.z;
+      tile[rr][c4 + 3] = o.w;
+    }
+    __syncthreads();
+    // transposed quads: thread -> (column kk = tid / 16 + 16 j, rows 4 (tid % 16) ..)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kk = (tid >> 4) + 16 * j, r4 = 4 * (tid & 15);
+      const i64 k = k0 + kk, r = r0 + r4;
+      if (k < K && r < rows) {
+        const ushort4 o = {tile[r4][kk], tile[r4 + 1][kk], tile[r4 + 2][kk], tile[r4 + 3][kk]};
+        uint16_t* d = dstT + k * rows + r;
+        if (r + 3 < rows && rows % 4 == 0) {
+          *reinterpret_cast<ushort4*>(d) = o;
+        } else {
+          d[0] = o.x;
+          if (r + 1 < rows) d[1] = o.y;
+          if (r + 2 < rows) d[2] = o.z;
+          if (r + 3 < rows) d[3] = o.w;
+        }
+      }
+    }
+    __syncthreads();  // tile reused by the next tile of this workgroup
+  }
+  // (sum, min) partial of the workgroup
+  double v = ss;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = v;
+  __syncthreads();
+  const double tot = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();
+  red[tid] = (double)mn;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (tid < k) red[tid] = fmin(red[tid], red[tid + k]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    part[blockIdx.x] = tot;
+    part[gridDim.x + blockIdx.x] = red[0];
+  }
+}
+
+int prepare_bf16_als(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, uint16_t* dst, uint16_t* dstT,
+                     double* tmp, int* nblk_out) {
+  i64 total = 1;
+  for (int k = 0; k < nd; ++k) total *= shape[k];
+  const i64 K = nd >= 1 ? shape[nd - 1] : 1, rows = K ? total / K : 0;
+  *nblk_out = 0;
+  if (total == 0 || nd < 2 || K % 4 != 0 || (uintptr_t)X % 16 != 0) return TENRSO_ERR_UNSUPPORTED;
+  const i64 tiles = cdiv(rows, 64) * cdiv(K, 64);
+  const int nblk = (int)std::min<i64>(1024, tiles);
+  hipLaunchKernelGGL(prepare_bf16_kernel, dim3(nblk), dim3(256), 0, ctx->stream, X, rows, K, dst, dstT, tmp);
+  TENRSO_HIP_TRY(hipGetLastError());
+  *nblk_out = nblk;
+  return TENRSO_OK;
+}
+
 int transpose_last_to_front_u16(tenrso_ctx* ctx, const uint16_t* src, int nd, const i64* shape, uint16_t* dst) {
   i64 total = 1;
   for (int k = 0; k < nd; ++k) total *= shape[k];

@@ -280,6 +280,9 @@ def cp_als_with_init(tensor, init_factors, max_iters, tol, time_limit=None, cons
         raise ShapeMismatch(f"Number of factor matrices ({len(init_factors)}) must match tensor rank ({len(shape)})")
     rank = int(init_factors[0].shape[1])
     _validate(shape, rank, tol)
+    for k, f in enumerate(init_factors):
+        if tuple(int(d) for d in f.shape) != (shape[k], rank):
+            raise ShapeMismatch(f"Initial factor {k} has shape {tuple(f.shape)}, expected ({shape[k]}, {rank})")
     cons = constraints or CpConstraints()
     cons.regularization.validate()
     if cons.regularization.kind == "None" and cons.l2_reg < 0.0:
@@ -309,7 +312,15 @@ def cp_als_with_init(tensor, init_factors, max_iters, tol, time_limit=None, cons
         X = tensor.contiguous()
         if X.dtype != torch.float32 or not X.is_cuda:
             raise TypeError("device cp_als expects a float32 CUDA tensor")
-        F = [torch.as_tensor(f, device=X.device).float().contiguous().clone() for f in init_factors]
+        # the result factors: one copy of all initial factors (one launch), each
+        # factor a contiguous view of it
+        src = [torch.as_tensor(f, device=X.device).float().reshape(-1) for f in init_factors]
+        flat = torch.cat(src)
+        F, off = [], 0
+        for f in init_factors:
+            n = int(f.shape[0]) * rank
+            F.append(flat[off:off + n].view(int(f.shape[0]), rank))
+            off += n
         ctx = _lib.torch_context(X)
         ptrs = (C.c_void_p * len(F))(*[f.data_ptr() for f in F])
         rc = lib().tenrso_cp_als_f32(ctx.ptr, X.data_ptr(), len(shape), shp, ptrs, C.byref(opts), C.byref(res),
