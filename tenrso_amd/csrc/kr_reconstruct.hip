@@ -13,8 +13,10 @@ namespace tenrso {
 
 static inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); }
 
+// vec: K % 4 == 0 and A, B, C 16-byte aligned (checked on the host): 16-byte
+// loads and stores, else scalar
 __global__ __launch_bounds__(256) void khatri_rao_kernel(const float* __restrict__ A, const float* __restrict__ B,
-                                                         i64 I, i64 J, i64 K, float* __restrict__ C) {
+                                                         i64 I, i64 J, i64 K, float* __restrict__ C, int vec) {
   const i64 t = (i64)blockIdx.x * 256 + threadIdx.x;
   const i64 kq = (K + 3) / 4;
   if (t >= I * J * kq) return;
@@ -23,7 +25,7 @@ __global__ __launch_bounds__(256) void khatri_rao_kernel(const float* __restrict
   const float* a = A + ia * K;
   const float* b = B + ib * K;
   float* c = C + row * K;
-  if (k0 + 3 < K && (K % 4) == 0) {
+  if (vec && k0 + 3 < K) {
     const float4 av = *reinterpret_cast<const float4*>(a + k0);
     const float4 bv = *reinterpret_cast<const float4*>(b + k0);
     *reinterpret_cast<float4*>(c + k0) = make_float4(av.x * bv.x, av.y * bv.y, av.z * bv.z, av.w * bv.w);
@@ -35,7 +37,8 @@ __global__ __launch_bounds__(256) void khatri_rao_kernel(const float* __restrict
 int khatri_rao_f32(tenrso_ctx* ctx, const float* A, i64 I, const float* B, i64 J, i64 K, float* C) {
   const i64 n = I * J * ((K + 3) / 4);
   if (n == 0) return TENRSO_OK;
-  hipLaunchKernelGGL(khatri_rao_kernel, dim3(cdiv(n, 256)), dim3(256), 0, ctx->stream, A, B, I, J, K, C);
+  const int vec = K % 4 == 0 && (uintptr_t)A % 16 == 0 && (uintptr_t)B % 16 == 0 && (uintptr_t)C % 16 == 0;
+  hipLaunchKernelGGL(khatri_rao_kernel, dim3(cdiv(n, 256)), dim3(256), 0, ctx->stream, A, B, I, J, K, C, vec);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }

@@ -654,10 +654,11 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __res
   if (o64) o64[idx] = s;
 }
 
+// vec: src 16-byte and dst 8-byte aligned (checked on the host), else scalar
 __global__ __launch_bounds__(256) void f32_to_bf16_kernel(const float* __restrict__ src, i64 n,
-                                                          uint16_t* __restrict__ dst) {
+                                                          uint16_t* __restrict__ dst, int vec) {
   const i64 i4 = ((i64)blockIdx.x * 256 + threadIdx.x) * 4;
-  if (i4 + 3 < n) {
+  if (vec && i4 + 3 < n) {
     const floatx4 v = *reinterpret_cast<const floatx4*>(src + i4);
     ushort4 o;
     o.x = f32_to_bf16_rne(v[0]);
@@ -748,8 +749,9 @@ int convert_bf16(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, bo
   for (int k = 0; k < nd; ++k) total *= shape[k];
   if (total == 0) return TENRSO_OK;
   if (!last_to_front || nd < 2) {
+    const int vec = (uintptr_t)src % 16 == 0 && (uintptr_t)dst % 8 == 0;
     hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(cdiv(cdiv(total, 4), 256)), dim3(256), 0, ctx->stream, src, total,
-                       dst);
+                       dst, vec);
   } else {
     const i64 K = shape[nd - 1], rows = total / K;
     hipLaunchKernelGGL(transpose_bf16_kernel<float>, dim3(cdiv(K, 32), cdiv(rows, 32)), dim3(256), 0, ctx->stream,
@@ -976,7 +978,10 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
       kw = (int)cand;
       break;
     }
-  const bool mfma_ok = pl.n_oth >= 1 && pl.J > 0 && kw > 0 && !(bf16 && caseB) && I > 0;
+  // the MFMA kernels read X with 8- and 16-byte vector loads: X must be 16-byte
+  // aligned (any view at another offset takes the generic kernel)
+  const bool mfma_ok =
+      pl.n_oth >= 1 && pl.J > 0 && kw > 0 && !(bf16 && caseB) && I > 0 && (uintptr_t)Xuse % 16 == 0;
 
   float* partial = nullptr;
   int splits = 1;

@@ -245,6 +245,34 @@ def test_cp_als_bf16_fit_close():
     assert abs(cp.fit - info["fit"]) <= 1e-4
 
 
+@pytest.mark.parametrize("shape", [(40, 36, 32), (35, 33, 36), (24, 20, 30)])
+def test_cp_als_bf16_setup_paths(shape):
+    """The BF16 ALS input (bf16 copy, last-to-front bf16 copy, norm) comes from one
+    fused pass when the last mode is a multiple of 4 and X is 16-byte aligned, from
+    three separate passes otherwise.  A copy of X at an odd element offset takes
+    the separate passes; both runs must give the same factors bit for bit (the
+    norms differ only in FP64 summation order), and the fit must stay within the
+    bf16 input rounding of the FP64 oracle's."""
+    import torch
+    print(f"[case] cp_als bf16 setup paths {shape}", flush=True)
+    rng = np.random.default_rng(5)
+    X = synth.cp_tensor(synth.normal_factors(shape, 6, rng), noise=1e-2, rng=rng)
+    init = synth.init_factors(shape, 6, seed=2)
+    Xa = dev(X)
+    buf = torch.zeros(X.size + 1, dtype=torch.float32, device=Xa.device)
+    buf[1:] = Xa.reshape(-1)
+    Xm = buf[1:].view(shape)  # 4-byte offset: not 16-byte aligned
+    assert Xm.data_ptr() % 16 != 0
+    cpa = decomp.cp_als_with_init(Xa, [dev(f) for f in init], 8, 0.0, precision="bf16")
+    cpm = decomp.cp_als_with_init(Xm, [dev(f) for f in init], 8, 0.0, precision="bf16")
+    _, info = oracle.cp_als(X, init, 8, 0.0)
+    print(f"  fit aligned {cpa.fit:.10f} offset {cpm.fit:.10f} oracle {info['fit']:.10f}", flush=True)
+    for a, m in zip(cpa.factors, cpm.factors):
+        assert torch.equal(a, m)
+    assert abs(cpa.fit - cpm.fit) <= 1e-12
+    assert abs(cpa.fit - info["fit"]) <= 3e-3
+
+
 @pytest.mark.parametrize("shape", [(256, 256, 256), (512, 512, 128)])
 def test_c3_congruent_precision_vs_fp64(shape):
     """Ill-conditioned Grams (pairwise column congruence 0.9, noise 1e-4), rank 64,
