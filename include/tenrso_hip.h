@@ -190,6 +190,20 @@ int tenrso_cp_als_f32_host(tenrso_ctx* ctx, const float* X, int ndim, const int6
                            double* fit_history);
 
 /* ------------------------------------------------------------------------
+ * SVD-based initial factors: InitStrategy::Svd / Nnsvd / LeverageScore of
+ * initialize_factors (tenrso-decomp cp/helpers.rs:232-373), from the SVD of every
+ * mode-k unfolding, on the device: the left singular vectors and singular values
+ * are the eigenpairs of X_(k) X_(k)^T (FP64 Gram, one-workgroup Jacobi), each
+ * vector's largest-magnitude entry made positive.  Mode sizes <= 1024, rank 1..64.
+ * X: device tensor; factors: device buffers (shape[k] x rank, fp32).  seed drives
+ * the N(0, 0.01) draws of the columns past min(rank, I_k, product of the other
+ * modes), which the reference takes from an unseeded thread_rng.
+ * ------------------------------------------------------------------------ */
+enum tenrso_init_kind { TENRSO_INIT_SVD = 2, TENRSO_INIT_NNSVD = 3, TENRSO_INIT_LEVERAGE = 4 };
+int tenrso_cp_init_svd(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, int64_t rank, int kind,
+                       uint64_t seed, float* const* factors);
+
+/* ------------------------------------------------------------------------
  * Reverse mode.  Device pointers throughout; rank <= 64 for the FP64 Gram paths.
  * ------------------------------------------------------------------------ */
 /* VJP of mttkrp (no reference counterpart; tenrso-ad registers custom rules through

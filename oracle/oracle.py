@@ -10,7 +10,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 __all__ = [
     "lib", "mttkrp", "mttkrp_fast", "mttkrp_blocked_parallel", "khatri_rao", "gram", "gram_hadamard",
     "solve_ls", "tikhonov", "soft_threshold", "orthonormalize", "norm_sq", "recon_norm_sq", "fit",
-    "cp_als", "cp_reconstruct", "extract_weights", "cp_factor_grads", "coo_mttkrp", "num_threads",
+    "cp_als", "cp_reconstruct", "extract_weights", "cp_factor_grads", "coo_mttkrp", "num_threads", "svd_init",
     "OrcConstraints", "REASONS",
 ]
 
@@ -269,4 +269,50 @@ def coo_mttkrp(indices, values, shape, factors, mode):
     out = np.zeros((shape[mode], R))
     lib().orc_coo_mttkrp_f64(_p(idx), _p(vals), C.c_int64(vals.size), len(shape), _shape(shape), _ptrs(F),
                              C.c_int64(R), mode, _p(out))
+    return out
+
+
+def svd_init(X, rank, kind):
+    """initialize_factors InitStrategy::Svd / Nnsvd / LeverageScore
+    (tenrso-decomp cp/helpers.rs:232-373) in FP64, with numpy's economy SVD of each
+    mode-k unfolding in place of the third-party one (`svd(&unfolded, false, None)`).
+
+    The SVD's signs are arbitrary; here, as on the device, each u_r is turned so
+    that its first entry of largest magnitude is positive (v_r with it).  Nnsvd
+    does not depend on that choice.  Columns past min(rank, I_k, J_k) are NaN: the
+    reference fills them from an unseeded thread_rng.
+    kind: "svd", "nnsvd" or "leverage_score".
+    """
+    X = np.asarray(X, dtype=np.float64)
+    out = []
+    for k, n in enumerate(X.shape):
+        Xk = np.moveaxis(X, k, 0).reshape(n, -1)
+        U, s, Vt = np.linalg.svd(Xk, full_matrices=False)
+        ar = min(rank, U.shape[1], Vt.shape[0])
+        for r in range(ar):
+            i = int(np.argmax(np.abs(U[:, r])))
+            if U[i, r] < 0:
+                U[:, r] = -U[:, r]
+                Vt[r] = -Vt[r]
+        F = np.full((n, rank), np.nan)
+        if kind == "svd":
+            F[:, :ar] = U[:, :ar]
+        elif kind == "nnsvd":
+            for r in range(ar):
+                u, v = U[:, r], Vt[r]
+                up, un = np.where(u > 0, u, 0.0), np.where(u > 0, 0.0, -u)   # split_sign (helpers.rs:380-398)
+                vp, vn = np.where(v > 0, v, 0.0), np.where(v > 0, 0.0, -v)
+                pp = np.linalg.norm(up) * np.linalg.norm(vp)
+                npr = np.linalg.norm(un) * np.linalg.norm(vn)
+                F[:, r] = up * np.sqrt(s[r] * pp) if pp >= npr else un * np.sqrt(s[r] * npr)
+        elif kind == "leverage_score":
+            lev = (U[:, :ar] ** 2).sum(axis=1) / ar
+            tot = lev.sum()
+            if tot > np.finfo(np.float32).eps:
+                lev = lev / tot
+            for r in range(ar):
+                F[:, r] = U[:, r] * np.sqrt(s[r]) * np.sqrt(lev * n)
+        else:
+            raise ValueError(kind)
+        out.append(F)
     return out

@@ -25,6 +25,7 @@ ERR_UNSUPPORTED = 9
 
 PREC_FP32 = 0
 PREC_BF16 = 1
+INIT_SVD, INIT_NNSVD, INIT_LEVERAGE = 2, 3, 4  # enum tenrso_init_kind
 
 REASONS = ["FitTolerance", "MaxIterations", "Oscillation", "TimeLimit"]
 
@@ -66,6 +67,7 @@ SIGNATURES = {
     "tenrso_cp_reconstruct_f32": ([_vp, _i, _vp, _vp, _i64, _vp, _vp], _i),
     "tenrso_cp_als_f32": ([_vp, _vp, _i, _vp, _vp, C.POINTER(CpOptions), C.POINTER(CpResult), _vp], _i),
     "tenrso_cp_als_f32_host": ([_vp, _vp, _i, _vp, _vp, C.POINTER(CpOptions), C.POINTER(CpResult), _vp], _i),
+    "tenrso_cp_init_svd": ([_vp, _vp, _i, _vp, _i64, _i, C.c_uint64, _vp], _i),
     "tenrso_mttkrp_vjp_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _i, _vp, _vp, _vp], _i),
     "tenrso_cp_factor_grads_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _vp, _vp], _i),
     "tenrso_cp_loss_grad_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _vp, C.POINTER(_d)], _i),

@@ -13,6 +13,7 @@
 #include "sparse.hpp"
 #include "als_job.hpp"
 #include "ad.hpp"
+#include "cp_init.hpp"
 
 using namespace tenrso;
 
@@ -304,6 +305,22 @@ int tenrso_cp_als_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* 
     return TENRSO_ERR_INVALID_ARGUMENT;
   }
   return cp_als_device(ctx, X, ndim, shape, factors, opts, result, fit_history);
+}
+
+int tenrso_cp_init_svd(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, int64_t rank, int kind,
+                       uint64_t seed, float* const* factors) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  if (!X || !shape || !factors || ndim <= 0) {
+    set_error("null tensor, shape or factors");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  if (rank <= 0) {
+    set_error("Invalid rank: %lld", (long long)rank);
+    return TENRSO_ERR_INVALID_RANK;
+  }
+  TENRSO_TRY(set_device(ctx));
+  return cp_init_svd(ctx, X, ndim, shape, rank, kind, seed, factors, nullptr);
 }
 
 int tenrso_cp_als_f32_host(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, float* const* factors,

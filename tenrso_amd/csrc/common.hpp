@@ -89,6 +89,10 @@ enum BufSlot : int {
   SLOT_SPM,
   SLOT_ORTHO,
   SLOT_FSET,
+  SLOT_EIG_A,
+  SLOT_EIG_V,
+  SLOT_EIG_PART,
+  SLOT_EIG_W,
   SLOT_COUNT
 };
 

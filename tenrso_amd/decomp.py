@@ -233,17 +233,48 @@ class CpDecomp:
 
 
 # ---------------------------------------------------------------- entry points
-def initialize_factors(shape, rank, init=InitStrategy.Random, seed=None):
-    """initialize_factors (cp/helpers.rs:192-377) for Random / RandomNormal.
+_SVD_KINDS = {"svd": _lib.INIT_SVD, "nnsvd": _lib.INIT_NNSVD, "leverage_score": _lib.INIT_LEVERAGE}
 
-    Svd, Nnsvd and LeverageScore (SVD of every unfolding) are not provided.
+
+def initialize_factors(shape, rank, init=InitStrategy.Random, seed=None, tensor=None):
+    """initialize_factors (cp/helpers.rs:192-377).
+
+    Random / RandomNormal are drawn here (seeded; the reference uses an unseeded
+    thread_rng).  Svd / Nnsvd / LeverageScore need the tensor: the SVD of every
+    unfolding runs on the device (tenrso_cp_init_svd; mode sizes <= 1024, rank
+    <= 64).  Returns float32 numpy arrays.
     """
     rng = np.random.default_rng(seed)
     if init == InitStrategy.Random:
         return [rng.random((n, rank)).astype(np.float32) for n in shape]
     if init == InitStrategy.RandomNormal:
         return [rng.standard_normal((n, rank)).astype(np.float32) for n in shape]
-    raise NotImplementedError(f"{init} initialisation is not provided; pass explicit factors to cp_als_with_init")
+    if init.value in _SVD_KINDS:
+        if tensor is None:
+            raise ValueError(f"{init} initialisation needs the tensor")
+        import torch
+        X = tensor if type(tensor).__module__.startswith("torch") else torch.from_numpy(
+            np.ascontiguousarray(tensor, dtype=np.float32))
+        if not X.is_cuda:
+            X = X.to("cuda")
+        X = X.float().contiguous()
+        if tuple(X.shape) != tuple(shape):
+            raise ShapeMismatch(f"tensor shape {tuple(X.shape)} does not match {tuple(shape)}")
+        F = [torch.empty((int(n), int(rank)), dtype=torch.float32, device=X.device) for n in shape]
+        ctx = _lib.torch_context(X)
+        shp = (C.c_int64 * len(shape))(*[int(n) for n in shape])
+        ptrs = (C.c_void_p * len(F))(*[f.data_ptr() for f in F])
+        seed_u = int(rng.integers(0, 2**63)) if seed is None else int(seed)
+        rc = lib().tenrso_cp_init_svd(ctx.ptr, X.data_ptr(), len(shape), shp, int(rank), _SVD_KINDS[init.value],
+                                      seed_u, ptrs)
+        if rc != _lib.OK:
+            msg = _lib.last_error()
+            mk = _ERRORS.get(rc)
+            if mk:
+                raise mk(msg)
+            raise _lib.TenrsoError(rc, msg)
+        return [f.cpu().numpy() for f in F]
+    raise ValueError(f"unknown initialisation {init}")
 
 
 def _validate(shape, rank, tol):
@@ -349,7 +380,7 @@ def cp_als(tensor, rank, max_iters, tol, init=InitStrategy.Random, time_limit=No
     """tenrso_decomp::cp_als (cp/core.rs:59-182)."""
     shape = tuple(int(s) for s in tensor.shape)
     _validate(shape, int(rank), tol)
-    F = initialize_factors(shape, int(rank), init, seed)
+    F = initialize_factors(shape, int(rank), init, seed, tensor=tensor)
     return cp_als_with_init(tensor, F, max_iters, tol, time_limit=time_limit, precision=precision)
 
 
@@ -358,6 +389,6 @@ def cp_als_constrained(tensor, rank, max_iters, tol, init=InitStrategy.Random, c
     """tenrso_decomp::cp_als_constrained (cp/core.rs:225-408)."""
     shape = tuple(int(s) for s in tensor.shape)
     _validate(shape, int(rank), tol)
-    F = initialize_factors(shape, int(rank), init, seed)
+    F = initialize_factors(shape, int(rank), init, seed, tensor=tensor)
     return cp_als_with_init(tensor, F, max_iters, tol, time_limit=time_limit, constraints=constraints,
                             precision=precision)

@@ -327,3 +327,44 @@ def test_orthonormalize_is_the_positive_diagonal_qr(orc):
         assert np.all(np.diag(Rm) > 0)
         L = np.linalg.cholesky(A.T @ A)
         assert np.allclose(Q, A @ np.linalg.inv(L).T, atol=1e-9)
+
+
+def test_svd_init_closed_forms(orc):
+    """The oracle's Svd / Nnsvd / LeverageScore inits (helpers.rs:232-373) on a
+    positive rank-1 tensor a o b o c, where the SVD of every unfolding is known:
+    u = a/|a|, s = |a||b||c|, v = (b (x) c)/(|b||c|).  Svd gives u; Nnsvd (both
+    sign parts of u and v are the vectors themselves) u sqrt(s); LeverageScore
+    (rank 1: l_i = u_i^2, already summing to 1) u |u| sqrt(s I)."""
+    rng = np.random.default_rng(0)
+    vecs = [rng.random(n) + 0.1 for n in (7, 6, 5)]
+    X = np.einsum("i,j,k->ijk", *vecs)
+    s = np.prod([np.linalg.norm(v) for v in vecs])
+    for kind in ("svd", "nnsvd", "leverage_score"):
+        F = orc.svd_init(X, 1, kind)
+        for k, v in enumerate(vecs):
+            u = v / np.linalg.norm(v)
+            want = {"svd": u, "nnsvd": u * np.sqrt(s), "leverage_score": u * np.abs(u) * np.sqrt(s * len(v))}[kind]
+            assert np.abs(F[k][:, 0] - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_svd_init_matches_eigenvectors_of_the_unfolding_gram(orc):
+    """Svd columns are the top eigenvectors of X_(k) X_(k)^T (numpy eigh, a different
+    LAPACK route than the SVD), orthonormal, largest-magnitude entry positive;
+    Nnsvd columns are non-negative; columns past min(R, I_k, J_k) are left to the
+    caller's draws (NaN here)."""
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((9, 8, 7))
+    F = orc.svd_init(X, 4, "svd")
+    for k, n in enumerate(X.shape):
+        Xk = np.moveaxis(X, k, 0).reshape(n, -1)
+        w, V = np.linalg.eigh(Xk @ Xk.T)
+        top = V[:, ::-1][:, :4]
+        for r in range(4):
+            col = F[k][:, r]
+            assert abs(abs(col @ top[:, r]) - 1.0) < 1e-10
+            assert col[np.argmax(np.abs(col))] > 0
+        assert np.allclose(F[k].T @ F[k], np.eye(4), atol=1e-12)
+    G = orc.svd_init(X, 4, "nnsvd")
+    assert all((g >= 0).all() for g in G)
+    H = orc.svd_init(rng.standard_normal((3, 5, 4)), 4, "svd")
+    assert np.isnan(H[0][:, 3]).all() and not np.isnan(H[0][:, :3]).any()
