@@ -51,7 +51,14 @@ struct AlsJob {
   int status_reset;      // write path instead of max(previous, path)
   double* host;          // with do_fit: mapped host copy of scal[0..JOB_SC_COUNT) (or null)
   double stamp;          // written last into host[JOB_SC_STAMP]
+  unsigned* gate;        // or null: set to gate_stamp once the Gram loads have returned
+  unsigned gate_stamp;
 };
+
+// Polls of the start gate by an MTTKRP workgroup beside the job (each poll is
+// a memory round trip, then a short sleep): the wait is bounded, so a gate
+// that never opens delays the launch by a few microseconds and nothing else.
+constexpr int JOB_GATE_POLLS = 12;
 
 __device__ __forceinline__ double job_recip(double p) {
   double r = __builtin_amdgcn_rcp(p);
@@ -327,6 +334,13 @@ __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
     fold(k + 1, g1);
     fold(k + 2, g2);
     fold(k + 3, g3);
+  }
+  if (a.gate) {
+    // the Grams (and <X, Xhat> partials) of this wave are in registers: open
+    // the gate of the MTTKRP workgroups, whose stream of X would otherwise
+    // queue ahead of these round trips
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    if (threadIdx.x == 0) __hip_atomic_store(a.gate, a.gate_stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (want_fit) {
     // ||Xhat||^2 = sum_{r,s} prod_k G_k[r,s]; <X, Xhat> = sum of the partials

@@ -60,6 +60,8 @@ struct tenrso_ctx {
   int profile_reps = 0;               // > 0: mttkrp_dense times its main kernel
   int wg_per_cu_f32 = 0;              // MTTKRP workgroups per CU; 0: the kernel's occupancy
   int wg_per_cu_bf16 = 0;             // (TENRSO_WG_PER_CU_F32 / _BF16 override)
+  int job_gate = 1;                   // BF16 MTTKRP waits for the job's Gram loads (TENRSO_JOB_GATE=0: off)
+  unsigned gate_stamp = 0;            // value the next gated launch waits for
   std::vector<tenrso::DevBuf> bufs;  // indexed by slot id
   void* pinned = nullptr;            // small pinned (coherent) host buffer for scalar readback
   double* pinned_dev = nullptr;      // the same buffer as a device pointer (kernels write the fit there)
@@ -93,6 +95,7 @@ enum BufSlot : int {
   SLOT_EIG_V,
   SLOT_EIG_PART,
   SLOT_EIG_W,
+  SLOT_GATE,
   SLOT_COUNT
 };
 

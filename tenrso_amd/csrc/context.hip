@@ -103,6 +103,7 @@ int tenrso_ctx_create(int device, int borrow_stream, void* stream, tenrso_ctx** 
   c->bufs.resize(SLOT_COUNT);
   if (const char* e = std::getenv("TENRSO_WG_PER_CU_F32")) c->wg_per_cu_f32 = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("TENRSO_WG_PER_CU_BF16")) c->wg_per_cu_bf16 = std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("TENRSO_JOB_GATE")) c->job_gate = std::atoi(e) != 0;
   *out = c;
   return TENRSO_OK;
 }

@@ -483,6 +483,16 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
     if (threadIdx.x < JOB_THREADS) als_job_run(job, reinterpret_cast<double*>(smem));
     return;
   }
+  if (has_job && job.gate) {
+    // start gate: wait (a bounded number of polls) until the job has its Grams
+    if (threadIdx.x == 0) {
+      for (int it = 0; it < JOB_GATE_POLLS; ++it) {
+        if (__hip_atomic_load(job.gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == job.gate_stamp) break;
+        __builtin_amdgcn_s_sleep(8);
+      }
+    }
+    __syncthreads();
+  }
   const int tile = (int)blockIdx.x - (has_job ? 1 : 0);
   const int xt = tile % a.rtiles;
   uint16_t* sHi = reinterpret_cast<uint16_t*>(smem);
@@ -1075,11 +1085,25 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     partial = (float*)pbuf;
     if (job && !fuse_job) TENRSO_TRY(als_job_launch(ctx, *job));
     const dim3 block(waves * 64);
+    // BF16 with the job fused: the MTTKRP workgroups wait for the job's Gram
+    // loads (the job, not the HBM-bound MTTKRP, sets the length of that launch)
+    unsigned* gate = nullptr;
+    if (bf16 && fuse_job && ctx->job_gate) {
+      void* gp = nullptr;
+      TENRSO_TRY(ctx_buffer(ctx, SLOT_GATE, sizeof(unsigned), &gp));
+      gate = (unsigned*)gp;
+    }
     auto launch_main = [&](bool with_job) -> int {
       for (int rc = 0; rc < rchunks; ++rc) {
         const int has_job = (with_job && fuse_job && rc == 0) ? 1 : 0;
         AlsJob jarg{};
-        if (has_job) jarg = *job;
+        if (has_job) {
+          jarg = *job;
+          if (gate) {
+            jarg.gate = gate;
+            jarg.gate_stamp = ++ctx->gate_stamp;
+          }
+        }
         const dim3 grid((unsigned)(rtiles * splits + has_job), 1, 1);
         a.r0 = rc * RT;
         a.rcount = (int)std::min<i64>(RT, R - a.r0);

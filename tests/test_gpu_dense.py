@@ -304,3 +304,34 @@ def test_c3_congruent_precision_vs_fp64(shape):
     assert e32 <= 1e-5 and e16 <= 5e-3
     assert results["fp32"][0] <= 1e-5 and results["fp32"][1] <= 1e-2
     assert results["bf16"][0] <= 1e-3 and results["bf16"][1] <= 0.5
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (33, 50, 70)])
+def test_cp_als_ridge_retry_vs_oracle(shape, precision):
+    """A zero column in the initial factors makes every Gamma_n singular (a zero
+    row and column), so every solve fails its first factorisation and takes the
+    ridge retry of helpers.rs:477-490 (Gamma + eps 10R I).  The zero column
+    decouples from the others, so the solution does not depend on eps: column 3
+    stays exactly zero and the rest must match the FP64 oracle, which takes the
+    same retry (solve path 1 on both sides)."""
+    print(f"[case] cp_als ridge retry {shape} {precision}", flush=True)
+    rng = np.random.default_rng(11)
+    X = synth.cp_tensor(synth.normal_factors(shape, 6, rng), noise=1e-2, rng=rng)
+    init = synth.init_factors(shape, 6, seed=4)
+    for f in init:
+        f[:, 3] = 0.0
+    cp = decomp.cp_als_with_init(dev(X), [dev(f) for f in init], 6, 0.0, precision=precision)
+    ref_F, info = oracle.cp_als(X, init, 6, 0.0)
+    print(f"  fit {cp.fit:.8f} oracle {info['fit']:.8f} paths {cp.solve_path_max} / {info['solve_path_max']}",
+          flush=True)
+    assert info["solve_path_max"] == 1
+    assert cp.solve_path_max == 1
+    bar = 1e-4 if precision == "fp32" else 3e-3
+    assert abs(cp.fit - info["fit"]) <= bar
+    for g, r in zip(cp.factors, ref_F):
+        g = g.cpu().numpy()
+        assert not g[:, 3].any()
+        if precision == "fp32":
+            keep = [c for c in range(6) if c != 3]
+            assert rel(g[:, keep], r[:, keep]) < 1e-3
