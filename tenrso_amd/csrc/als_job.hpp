@@ -391,39 +391,49 @@ __device__ __forceinline__ void als_job_run(const AlsJob& a, double* lds) {
 #pragma unroll
     for (int e = 0; e < 4; ++e) a.Winv[job_row(e) * JOB_NE + job_col(t)] = gam[t][e];
   doublex4 acc[4];
-  // Gamma + regularisation, identity outside R x R; the retry (ridge) reads Gamma
-  // back from Winv, so gam is dead once the first attempt's matrix is formed
-  auto form = [&](bool retry) {
-    const double ridge = retry ? (double)FLT_EPSILON * (double)(10 * R) : 0.0;
-    // uniform branch: without any regularisation term the band arithmetic of
-    // job_reg is not executed at all
-    if (retry || a.tik_order > 0 || a.l2 > 0.0) {
+  // acc = Gamma (first attempt) or Gamma from Winv (retry), then the identity
+  // outside R x R and the regularisation band, each in a uniform branch taken
+  // only when it changes something.  The retry is written out as a second
+  // attempt rather than a loop: in a loop the compiler hoists the band
+  // comparisons of the retry into the loop header, where the first attempt
+  // pays for them.
+  auto pad_and_band = [&](double ridge) {
+    if (R < JOB_NE) {
 #pragma unroll
       for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int i = job_row(e), col = job_col(t);
-          const double g = retry ? a.Winv[i * JOB_NE + col] : gam[t][e];
-          acc[t][e] = (i < R && col < R) ? g + job_reg(a, i, col, ridge) : (i == col ? 1.0 : 0.0);
+          acc[t][e] = (i < R && col < R) ? acc[t][e] : (i == col ? 1.0 : 0.0);
         }
-    } else {
+    }
+    if (ridge > 0.0 || a.tik_order > 0 || a.l2 > 0.0) {
 #pragma unroll
       for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int i = job_row(e), col = job_col(t);
-          acc[t][e] = (i < R && col < R) ? gam[t][e] : (i == col ? 1.0 : 0.0);
+          if (i < R && col < R) acc[t][e] += job_reg(a, i, col, ridge);
         }
     }
   };
-  form(false);
-#pragma unroll 1
-  for (int attempt = 0; attempt < 2 && !ok; ++attempt) {
-    if (attempt) form(true);
-    TENRSO_JOB_MARK(3);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[t][e] = gam[t][e];
+  pad_and_band(0.0);
+  TENRSO_JOB_MARK(3);
+  ok = job_block_gj(acc, lds);
+  TENRSO_JOB_MARK(4);
+  if (!ok) {
+    // ridge retry (cp/helpers.rs:477-490): Gamma + eps 10R I
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[t][e] = a.Winv[job_row(e) * JOB_NE + job_col(t)];
+    pad_and_band((double)FLT_EPSILON * (double)(10 * R));
     ok = job_block_gj(acc, lds);
-    TENRSO_JOB_MARK(4);
-    path = attempt;
+    path = 1;
   }
   if (!ok) path = 2;
 #pragma unroll
