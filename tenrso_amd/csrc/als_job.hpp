@@ -108,49 +108,13 @@ __device__ __forceinline__ double job_reg(const AlsJob& a, int i, int j, double 
 
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-// Inverse of a symmetric 2 x 2 matrix [[a, b], [b, d]]; ok stays true only
-// if it is positive definite.
-__device__ __forceinline__ void job_inv2(double a, double b, double d, double& ia, double& ib, double& id, bool& ok) {
-  const double det = fma(a, d, -(b * b));
-  ok = ok && a > 0.0 && det > 0.0 && det <= DBL_MAX;  // NaN fails every comparison
-  const double r = job_recip(det);
-  ia = d * r;
-  ib = -b * r;
-  id = a * r;
-}
-
-// Inverse of a symmetric positive definite 4 x 4 block P = [[A, B], [B^T, D]]
-// (2 x 2 blocks) through the Schur complement S = D - B^T A^-1 B:
-//   P^-1 = [[A^-1 + Y X^T, -Y], [-Y^T, S^-1]],  X = A^-1 B,  Y = X S^-1.
-// ok becomes false if A or S is not positive definite (or not finite).
-__device__ __forceinline__ void job_inv4_spd(const double (&p)[4][4], double (&o)[4][4], bool& ok) {
-  double a00, a01, a11;
-  job_inv2(p[0][0], p[0][1], p[1][1], a00, a01, a11, ok);
-  const double x00 = fma(a00, p[0][2], a01 * p[1][2]), x01 = fma(a00, p[0][3], a01 * p[1][3]);
-  const double x10 = fma(a01, p[0][2], a11 * p[1][2]), x11 = fma(a01, p[0][3], a11 * p[1][3]);
-  const double s00 = p[2][2] - fma(p[0][2], x00, p[1][2] * x10);
-  const double s01 = p[2][3] - fma(p[0][2], x01, p[1][2] * x11);
-  const double s11 = p[3][3] - fma(p[0][3], x01, p[1][3] * x11);
-  double i00, i01, i11;
-  job_inv2(s00, s01, s11, i00, i01, i11, ok);
-  const double y00 = fma(x00, i00, x01 * i01), y01 = fma(x00, i01, x01 * i11);
-  const double y10 = fma(x10, i00, x11 * i01), y11 = fma(x10, i01, x11 * i11);
-  o[0][0] = fma(y00, x00, fma(y01, x01, a00));
-  o[0][1] = fma(y00, x10, fma(y01, x11, a01));
-  o[1][1] = fma(y10, x10, fma(y11, x11, a11));
-  o[1][0] = o[0][1];
-  o[0][2] = -y00;
-  o[0][3] = -y01;
-  o[1][2] = -y10;
-  o[1][3] = -y11;
-  o[2][0] = -y00;
-  o[3][0] = -y01;
-  o[2][1] = -y10;
-  o[3][1] = -y11;
-  o[2][2] = i00;
-  o[2][3] = i01;
-  o[3][2] = i01;
-  o[3][3] = i11;
+// v from the lane of the same quad given by the quad permutation `ctrl`
+// (0xB1: lane ^ 1, 0x4E: lane ^ 2), one DPP move per 32-bit half.
+template <int CTRL>
+__device__ __forceinline__ double job_quad_perm(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
 }
 
 // Matrix layout of the job: wave w (of 4) holds rows 16w..16w+15 of the
@@ -179,7 +143,7 @@ __device__ __forceinline__ int job_col(int t) { return 16 * t + (threadIdx.x & 1
 // layout of the 16x16x4 instruction) and updates its four tiles with four
 // v_mfma_f64_16x16x4f64, the tile holding the next pivot block first.  The
 // wave holding the next pivot rows publishes them (one 8-byte store per lane
-// and tile) and one of its lanes inverts the next P (job_inv4_spd) while the
+// and tile) and 16 of its lanes invert the next P (adjugate, below) while the
 // remaining MFMAs run.  LDS is double-buffered: one barrier per step.  Returns
 // false if a pivot block is not positive definite.
 __device__ __forceinline__ bool job_block_gj(doublex4 (&acc)[4], double* lds) {
@@ -187,27 +151,47 @@ __device__ __forceinline__ bool job_block_gj(doublex4 (&acc)[4], double* lds) {
   const int q = lane >> 4, j = lane & 15;
   int* flag = reinterpret_cast<int*>(lds + JOB_GJ_FLAG);
   if (threadIdx.x == 0) *flag = 0;
-  // lane 0 of the wave holding rows 4s..4s+3: P from the published rows -> P^-1
+  // Inverse of the pivot block P = A[J][J] by the wave holding rows J, from the
+  // published rows: lane (r, c) = (lane / 4 % 4, lane % 4) forms the cofactor
+  // C[r][c] from its 3 x 3 minor (nine LDS reads at offsets fixed per lane);
+  // det = sum_c P[r][c] C[r][c], summed over the lane's quad; lanes 0..15 write
+  // P^-1[c][r] = C[r][c] / det.  Positive definiteness is Sylvester's criterion:
+  // p00, p00 p11 - p01^2, C[3][3] (the leading 3 x 3 minor) and det all > 0
+  // (NaN fails every comparison).  Sixteen lanes share what one lane did
+  // serially (a 4 x 4 Schur-complement inverse, about 50 dependent FP64 steps).
+  const int lr = (lane >> 2) & 3, lc = lane & 3;
+  int moff[9];
+  {
+    const int rr[3] = {lr == 0 ? 1 : 0, lr <= 1 ? 2 : 1, lr <= 2 ? 3 : 2};
+    const int cc[3] = {lc == 0 ? 1 : 0, lc <= 1 ? 2 : 1, lc <= 2 ? 3 : 2};
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+#pragma unroll
+      for (int v = 0; v < 3; ++v) moff[3 * u + v] = rr[u] * JOB_NE + cc[v];
+  }
+  const int self_off = lr * JOB_NE + lc;
+  const double csign = ((lr + lc) & 1) ? -1.0 : 1.0;
   auto invert_pivot = [&](int s) {
-    const double* Rw = lds + JOB_GJ_ROW + (s & 1) * 4 * JOB_NE;
+    const double* Pb = lds + JOB_GJ_ROW + (s & 1) * 4 * JOB_NE + 4 * s;  // P[u][v] = Pb[u * JOB_NE + v]
     double* Pi = lds + JOB_GJ_PINV + (s & 1) * 16;
-    double p[4][4], pinv[4][4];
+    double m[9];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const double2 lo = *reinterpret_cast<const double2*>(&Rw[u * JOB_NE + 4 * s]);
-      const double2 hi = *reinterpret_cast<const double2*>(&Rw[u * JOB_NE + 4 * s + 2]);
-      p[u][0] = lo.x;
-      p[u][1] = lo.y;
-      p[u][2] = hi.x;
-      p[u][3] = hi.y;
-    }
-    bool ok = true;
-    job_inv4_spd(p, pinv, ok);
-    if (!ok) *flag = 1;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      *reinterpret_cast<double2*>(&Pi[r * 4]) = make_double2(pinv[r][0], pinv[r][1]);
-      *reinterpret_cast<double2*>(&Pi[r * 4 + 2]) = make_double2(pinv[r][2], pinv[r][3]);
+    for (int k = 0; k < 9; ++k) m[k] = Pb[moff[k]];
+    const double pself = Pb[self_off];
+    const double p00 = Pb[0], p01 = Pb[1], p11 = Pb[JOB_NE + 1];
+    const double d3 = m[0] * fma(m[4], m[8], -(m[5] * m[7])) - m[1] * fma(m[3], m[8], -(m[5] * m[6])) +
+                      m[2] * fma(m[3], m[7], -(m[4] * m[6]));
+    const double cof = csign * d3;
+    double det = pself * cof;
+    det += job_quad_perm<0xB1>(det);
+    det += job_quad_perm<0x4E>(det);
+    const double det2 = fma(p00, p11, -(p01 * p01));
+    bool bad = !(p00 > 0.0 && det2 > 0.0 && det > 0.0 && det <= DBL_MAX);
+    if (lane == 15) bad = bad || !(cof > 0.0);
+    const double rd = job_recip(det);
+    if (lane < 16) {
+      Pi[lc * 4 + lr] = cof * rd;
+      if (bad) *flag = 1;
     }
   };
   // step 0's pivot rows and inverse
@@ -215,7 +199,7 @@ __device__ __forceinline__ bool job_block_gj(doublex4 (&acc)[4], double* lds) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) lds[JOB_GJ_ROW + q * JOB_NE + 16 * t + j] = acc[t][0];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    if (lane == 0) invert_pivot(0);
+    invert_pivot(0);
   }
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
@@ -260,7 +244,7 @@ __device__ __forceinline__ bool job_block_gj(doublex4 (&acc)[4], double* lds) {
       if (wave == T1) {
         Rn[q * JOB_NE + 16 * T1 + j] = acc[T1][e1];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        if (lane == 0) invert_pivot(s1);
+        invert_pivot(s1);
 #pragma unroll
         for (int i = 1; i < 4; ++i) {
           const int t = (T1 + i) & 3;
