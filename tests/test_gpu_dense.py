@@ -335,3 +335,32 @@ def test_cp_als_ridge_retry_vs_oracle(shape, precision):
         if precision == "fp32":
             keep = [c for c in range(6) if c != 3]
             assert rel(g[:, keep], r[:, keep]) < 1e-3
+
+
+def test_cp_als_bf16_start_gate_changes_timing_only(monkeypatch):
+    """The start gate of the fused BF16 launch (the MTTKRP workgroups wait for the
+    FP64 job's Gram loads) only delays work: factors and fit are the same bit for
+    bit with TENRSO_JOB_GATE=1 and =0.  The variable is read when a context is
+    created, so each run gets a new stream (and with it a new context)."""
+    import torch
+    print("[case] cp_als bf16 start gate on/off (96, 64, 64) rank 16", flush=True)
+    shape = (96, 64, 64)
+    rng = np.random.default_rng(21)
+    X = synth.cp_tensor(synth.normal_factors(shape, 16, rng), noise=1e-2, rng=rng)
+    init = synth.init_factors(shape, 16, seed=6)
+    Xd = dev(X)
+    Fd = [dev(f) for f in init]
+    runs = []
+    for gate in ("1", "0"):
+        monkeypatch.setenv("TENRSO_JOB_GATE", gate)
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            cp = decomp.cp_als_with_init(Xd, Fd, 8, 0.0, precision="bf16")
+            facs = [f.clone() for f in cp.factors]
+        s.synchronize()
+        runs.append((cp.fit, facs))
+    print(f"  fit gate on {runs[0][0]:.10f} off {runs[1][0]:.10f}", flush=True)
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert torch.equal(a, b)
