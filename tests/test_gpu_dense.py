@@ -364,3 +364,35 @@ def test_cp_als_bf16_start_gate_changes_timing_only(monkeypatch):
     assert runs[0][0] == runs[1][0]
     for a, b in zip(runs[0][1], runs[1][1]):
         assert torch.equal(a, b)
+
+
+ALS_NWAY = [((64, 48), 5), ((96, 32), 8), ((16, 20, 12, 32), 6), ((6, 5, 8, 7, 32), 3), ((4, 5, 3, 6, 4, 32), 3),
+            ((30, 20, 25), 1)]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("shape,rank", ALS_NWAY)
+def test_cp_als_nway_vs_oracle(shape, rank, precision):
+    """cp_als on 2-, 4-, 5- and 6-way tensors and at rank 1 (the ALS job loads the Grams
+    of up to four modes per round trip, so five and more modes take a second group;
+    a 2-way Gamma is a single Gram).  FP32: fit history within 1e-4 of the FP64
+    oracle and factors within 1e-3.  BF16 computes with X rounded to bf16, so its
+    factors are checked against the FP64 oracle run on that rounded X (within 1e-3).
+    Its fit is not held to the FP64 one here: it takes ||X||^2 from the fp32
+    tensor and <X, Xhat> from the bf16 one, and on a tensor this small and this
+    well fitted (residual 1e-2) the rounding's cross term is as large as the
+    squared residual itself (the 2-way case clamps at 1.0)."""
+    print(f"[case] cp_als {len(shape)}-way {shape} rank {rank} {precision}", flush=True)
+    rng = np.random.default_rng(len(shape) * 7 + rank)
+    X = synth.cp_tensor(synth.normal_factors(shape, rank, rng), noise=1e-2, rng=rng)
+    init = synth.init_factors(shape, rank, seed=3)
+    cp = decomp.cp_als_with_init(dev(X), [dev(f) for f in init], 8, 0.0, precision=precision)
+    Xref = X if precision == "fp32" else bf16_round(X)
+    ref_F, info = oracle.cp_als(Xref, init, 8, 0.0)
+    errs = [rel(g.cpu().numpy(), r) for g, r in zip(cp.factors, ref_F)]
+    print(f"  fit {cp.fit:.8f} oracle {info['fit']:.8f} iters {cp.iters}/{info['iters']} "
+          f"factor rel errs {['%.1e' % e for e in errs]}", flush=True)
+    assert cp.iters == info["iters"]
+    assert max(errs) < 1e-3
+    if precision == "fp32":
+        assert np.abs(np.array(cp.convergence.fit_history) - info["fit_history"]).max() <= 1e-4
