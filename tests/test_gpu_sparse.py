@@ -169,12 +169,14 @@ def test_sparse_mttkrp_deterministic():
     assert rel(a, oracle.coo_mttkrp(idx, vals, (1000, 800, 600), [f.astype(np.float64) for f in F], 0)) <= 1e-5
 
 
-def test_sparse_cp_als_vs_oracle():
-    print("[case] sparse cp_als 60x50x40", flush=True)
-    idx, vals = random_coo((60, 50, 40), 20000, seed=7)
-    shape = (60, 50, 40)
+@pytest.mark.parametrize("shape,nnz,R", [((60, 50, 40), 20000, 8), ((14, 12, 10, 9), 6000, 5), ((70, 50), 1500, 6)])
+def test_sparse_cp_als_vs_oracle(shape, nnz, R):
+    """Sparse CP-ALS (3-, 4- and 2-way) against the FP64 oracle's dense ALS on the
+    same tensor from the same initial factors: fit within 1e-4, same iterations."""
+    print(f"[case] sparse cp_als {shape} nnz {nnz} rank {R}", flush=True)
+    idx, vals = random_coo(shape, nnz, seed=7)
     sp = SparseTensor(idx, vals, shape)
-    init = synth.init_factors(shape, 8, seed=1)
+    init = synth.init_factors(shape, R, seed=1)
     cp = sp.cp_als_with_init(init, 10, 0.0)
     dense = CooTensor(idx, vals, shape).to_dense()
     _, info = oracle.cp_als(dense, init, 10, 0.0)
