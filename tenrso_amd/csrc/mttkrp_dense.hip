@@ -180,13 +180,15 @@ __device__ float kPreOnes[64] = {TENRSO_ONES8, TENRSO_ONES8, TENRSO_ONES8, TENRS
 // Case A (X as [P, I, Q], slow = p*qs + sq): the block starts at element
 // p*I*Q + sq*fast_size + w*kw + kb*KB.  Case B: the block covers j = slow*fast_size
 // + w*kw + kb*KB onwards, one row of I elements per j.  All fields are uniform.
-template <int WAVES, int KB, bool CASE_B>
+template <int STRIDE, int KB, bool CASE_B>
 struct BlockCursor {
   int kb, nkb;
   i64 slow, sq, base, wrap;
 
-  __device__ __forceinline__ BlockCursor(const KernArgs& a, i64 slow0, int w, int nkb_) {
-    kb = 0;
+  // starts at block kb0 of slow index slow0; each slow index has nkb blocks, and
+  // after its last one the cursor moves STRIDE slow indices on
+  __device__ __forceinline__ BlockCursor(const KernArgs& a, i64 slow0, int w, int nkb_, int kb0) {
+    kb = kb0;
     nkb = nkb_;
     slow = slow0 < a.n_slow ? slow0 : a.n_slow - 1;
     if constexpr (!CASE_B) {
@@ -209,10 +211,10 @@ struct BlockCursor {
   __device__ __forceinline__ void step(const KernArgs& a) {
     if (++kb == nkb) {
       kb = 0;
-      slow += WAVES;
-      base += (i64)WAVES * a.fast_size;
+      slow += STRIDE;
+      base += (i64)STRIDE * a.fast_size;
       if constexpr (!CASE_B) {
-        sq += WAVES;
+        sq += STRIDE;
         while (sq >= a.qs) {
           sq -= a.qs;
           base += wrap;
@@ -278,9 +280,9 @@ __device__ __forceinline__ void fold_slow(floatx4 (&acc)[MT][NT], floatx4 (&T)[M
 // ---------------------------------------------------------------------------
 // FP32 MFMA kernel (v_mfma_f32_16x16x4f32).
 //   A workgroup owns ROWS = 16*MT rows and one window w of the fastest mode; its
-//   WAVES waves share the LDS window and split the workgroup's slow range
-//   (wave k takes slow indices s_lo + k, s_lo + k + WAVES, ...); their
-//   accumulators are summed in LDS in fixed order at the end.
+//   WAVES waves share the LDS window and split the workgroup's blocks (slow
+//   index, block of the window within it) into contiguous balanced ranges;
+//   their accumulators are summed in LDS in fixed order at the end.
 //   The B operand of the MFMAs is the Khatri-Rao row itself, formed on the fly:
 //   the FAST window is read as 4 consecutive k from LDS and scaled by
 //   PRE[slow, r] of the lane's column, KR[j, r] = PRE[slow, r] * FAST[k, r]
@@ -326,8 +328,12 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
   const i64 s_lo = uniform_i64((i64)ss * a.slow_q + (ss < a.slow_rem ? ss : a.slow_rem));
   const i64 s_hi = s_lo + a.slow_q + (ss < a.slow_rem ? 1 : 0);
   const int nkb = a.kw / KB;
-  const i64 my_slow = (s_hi - s_lo > wave) ? (s_hi - s_lo - wave + WAVES - 1) / WAVES : 0;
-  const int T = __builtin_amdgcn_readfirstlane((int)(my_slow * nkb));  // blocks of this wave
+  // the workgroup's blocks (slow index, then block within it) in WAVES contiguous,
+  // balanced ranges: with 9 slow indices of 4 blocks, 9 blocks per wave rather
+  // than 12 for one wave and 8 for the others
+  const i64 nblk_wg = (s_hi - s_lo) * nkb;
+  const i64 wb0 = uniform_i64(wave * nblk_wg / WAVES), wb1 = (wave + 1) * nblk_wg / WAVES;
+  const int T = __builtin_amdgcn_readfirstlane((int)(wb1 - wb0));  // blocks of this wave
   const i64 row_base = (i64)xt * ROWS;
 
   // window loads first (see stage_window_load)
@@ -377,7 +383,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) poff[nt] = (has_pre ? a.r0 : 0) + (16 * nt + c < a.rcount ? 16 * nt + c : 0);
 
-  BlockCursor<WAVES, KB, CASE_B> cur(a, s_lo + wave, w, nkb);
+  BlockCursor<1, KB, CASE_B> cur(a, s_lo + wb0 / nkb, w, nkb, (int)(wb0 % nkb));
 
   auto load = [&](int t, Buf& b) {
     b.kb = cur.kb;
@@ -507,8 +513,12 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
   const i64 s_lo = uniform_i64((i64)ss * a.slow_q + (ss < a.slow_rem ? ss : a.slow_rem));
   const i64 s_hi = s_lo + a.slow_q + (ss < a.slow_rem ? 1 : 0);
   const int nkb = a.kw / KB;
-  const i64 my_slow = (s_hi - s_lo > wave) ? (s_hi - s_lo - wave + WAVES - 1) / WAVES : 0;
-  const int T = __builtin_amdgcn_readfirstlane((int)(my_slow * nkb));  // blocks of this wave
+  // the workgroup's blocks (slow index, then block within it) in WAVES contiguous,
+  // balanced ranges: with 9 slow indices of 4 blocks, 9 blocks per wave rather
+  // than 12 for one wave and 8 for the others
+  const i64 nblk_wg = (s_hi - s_lo) * nkb;
+  const i64 wb0 = uniform_i64(wave * nblk_wg / WAVES), wb1 = (wave + 1) * nblk_wg / WAVES;
+  const int T = __builtin_amdgcn_readfirstlane((int)(wb1 - wb0));  // blocks of this wave
   const i64 row_base = (i64)xt * ROWS;
 
   FastWindow<RT, NTH, KWMAX> fw;  // window loads before the X blocks
@@ -549,7 +559,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) poff[nt] = (has_pre ? a.r0 : 0) + (16 * nt + c < a.rcount ? 16 * nt + c : 0);
 
-  BlockCursor<WAVES, KB, false> cur(a, s_lo + wave, w, nkb);
+  BlockCursor<1, KB, false> cur(a, s_lo + wb0 / nkb, w, nkb, (int)(wb0 % nkb));
 
   auto load = [&](int t, Buf& b) {
     b.kb = cur.kb;
@@ -582,7 +592,8 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
           Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b.x[mt][ks], bl, Tacc[mt][nt], 0, 0, 0);
         }
       }
-    if (b.kb == nkb - 1) fold_slow<MT, NT>(acc, Tacc, b.pre);
+    // a slow index ends, or this wave's range does (it may end inside one)
+    if (b.kb == nkb - 1 || t == T - 1) fold_slow<MT, NT>(acc, Tacc, b.pre);
   };
 
   Buf b0, b1, b2, b3;
