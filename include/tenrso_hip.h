@@ -255,6 +255,15 @@ int tenrso_sp_mode_arrays(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, i
  * fptr/fids NULL to get nfib[l] (fibers per level, ndim entries). */
 int tenrso_sp_csf(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, int64_t* nfib, int64_t* const* fptr,
                   int32_t* const* fids, float* values);
+/* HiCOO of the COO as uploaded (HiCooTensor::from_coo, hicoo.rs:121-205): block
+ * coordinate = index / block_shape, local coordinate = index % block_shape,
+ * nonzeros stably sorted by block coordinate (lexicographic), duplicates kept.
+ * Call with the output pointers NULL to get *num_blocks (B).  block_coords is
+ * ndim x B (SoA: block_coords[k * B + b]), block_ptrs B + 1 (block b holds
+ * nonzeros block_ptrs[b]..block_ptrs[b+1]), local ndim x nnz (SoA), values nnz.
+ * A zero in block_shape is TENRSO_ERR_INVALID_ARGUMENT (hicoo.rs:139-143). */
+int tenrso_sp_hicoo(tenrso_ctx* ctx, const tenrso_sptensor* t, const int64_t* block_shape, int64_t* num_blocks,
+                    int64_t* block_coords, int64_t* block_ptrs, int32_t* local, float* values);
 /* Sparse MTTKRP (no reference counterpart; contract of the dense mttkrp on the
  * tensor the COO represents).  factors / out are device pointers. */
 int tenrso_sp_mttkrp_f32(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* factors, int64_t rank,

@@ -27,5 +27,8 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
   cp_factor_grads                      pinned: FP64 central differences, identity
                                        dL/dA_n = A_n G_n - M_n.
   coo_mttkrp                           pinned: COO -> dense + dense MTTKRP.
+  hicoo_from_coo                       pinned: hicoo.rs:414-575 unit cases (block
+                                       counts, coordinates, 5-way, empty, errors),
+                                       stable-order and round-trip invariants.
 """
 from .oracle import *  # noqa: F401,F403

@@ -11,6 +11,7 @@ __all__ = [
     "lib", "mttkrp", "mttkrp_fast", "mttkrp_blocked_parallel", "khatri_rao", "gram", "gram_hadamard",
     "solve_ls", "tikhonov", "soft_threshold", "orthonormalize", "norm_sq", "recon_norm_sq", "fit",
     "cp_als", "cp_reconstruct", "extract_weights", "cp_factor_grads", "coo_mttkrp", "num_threads", "svd_init",
+    "hicoo_from_coo",
     "OrcConstraints", "REASONS",
 ]
 
@@ -316,3 +317,27 @@ def svd_init(X, rank, kind):
             raise ValueError(kind)
         out.append(F)
     return out
+
+
+def hicoo_from_coo(indices, values, shape, block_shape):
+    """HiCooTensor::from_coo (tenrso-sparse/src/hicoo.rs:121-205) restated with numpy:
+    block = index // bs, local = index % bs, a stable sort of the nonzeros by block
+    coordinate (lexicographic; sort_by is stable, so a block keeps COO order),
+    block_ptrs at each change of block plus the final nnz.  Returns
+    (block_coords (ndim, B), block_ptrs (B + 1), local (ndim, nnz), values)."""
+    idx = np.asarray(indices, np.int64).reshape(len(shape), -1)
+    vals = np.asarray(values)
+    bs = np.asarray(block_shape, np.int64)
+    if bs.size != len(shape) or (bs <= 0).any():
+        raise ValueError("invalid block shape")
+    nnz = vals.size
+    if nnz == 0:
+        return np.zeros((len(shape), 0), np.int64), np.zeros(1, np.int64), np.zeros((len(shape), 0), np.int64), vals
+    blk = idx // bs[:, None]
+    loc = idx % bs[:, None]
+    order = np.lexsort(blk[::-1])  # lexsort is stable
+    sb, sl, sv = blk[:, order], loc[:, order], vals[order]
+    change = np.ones(nnz, bool)
+    change[1:] = (sb[:, 1:] != sb[:, :-1]).any(axis=0)
+    starts = np.nonzero(change)[0]
+    return sb[:, starts], np.append(starts, nnz).astype(np.int64), sl, sv

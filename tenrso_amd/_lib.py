@@ -76,6 +76,7 @@ SIGNATURES = {
     "tenrso_sp_destroy": ([_vp], _i),
     "tenrso_sp_mode_arrays": ([_vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
     "tenrso_sp_csf": ([_vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
+    "tenrso_sp_hicoo": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "tenrso_sp_mttkrp_f32": ([_vp, _vp, _vp, _i64, _i, _vp], _i),
     "tenrso_sp_cp_als_f32": ([_vp, _vp, _vp, C.POINTER(CpOptions), C.POINTER(CpResult), _vp], _i),
 }

@@ -422,6 +422,29 @@ int tenrso_sp_csf(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, int64_t* 
   return TENRSO_OK;
 }
 
+int tenrso_sp_hicoo(tenrso_ctx* ctx, const tenrso_sptensor* t, const int64_t* block_shape, int64_t* num_blocks,
+                    int64_t* block_coords, int64_t* block_ptrs, int32_t* local, float* values) {
+  clear_error();
+  if (bad_ctx(ctx) || !t || !block_shape || !num_blocks) return TENRSO_ERR_INVALID_ARGUMENT;
+  for (int k = 0; k < t->nd; ++k)
+    if (block_shape[k] <= 0) {
+      // hicoo.rs:139-143
+      set_error("Invalid block shape: block_shape cannot contain zeros");
+      return TENRSO_ERR_INVALID_ARGUMENT;
+    }
+  TENRSO_TRY(set_device(ctx));
+  std::vector<int64_t> bc, bp;
+  std::vector<int32_t> loc;
+  std::vector<float> vals;
+  TENRSO_TRY(sp_export_hicoo(ctx, t, block_shape, bc, bp, loc, vals));
+  *num_blocks = (int64_t)bp.size() - 1;
+  if (block_coords) std::memcpy(block_coords, bc.data(), bc.size() * sizeof(int64_t));
+  if (block_ptrs) std::memcpy(block_ptrs, bp.data(), bp.size() * sizeof(int64_t));
+  if (local) std::memcpy(local, loc.data(), loc.size() * sizeof(int32_t));
+  if (values) std::memcpy(values, vals.data(), vals.size() * sizeof(float));
+  return TENRSO_OK;
+}
+
 int tenrso_sp_mttkrp_f32(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* factors, int64_t rank,
                          int mode, float* out) {
   clear_error();

@@ -653,4 +653,130 @@ int sp_export_csf(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, std::vect
   return TENRSO_OK;
 }
 
+
+// ---------------------------------------------------------------- HiCOO
+struct HiCooArgs {
+  int nd;
+  i64 nnz;
+  i64 bs[MAXD_SP];   // block shape
+  i64 nbk[MAXD_SP];  // blocks per mode, ceil(shape / bs)
+  const int32_t* idx;
+};
+
+// key = block coordinates (index / bs) linearised, mode 0 most significant, so
+// the key order is the lexicographic order of the block coordinates
+__global__ __launch_bounds__(256) void hicoo_keys_kernel(HiCooArgs a, unsigned long long* __restrict__ keys,
+                                                         int32_t* __restrict__ pos) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= a.nnz) return;
+  unsigned long long k = 0;
+  for (int m = 0; m < a.nd; ++m)
+    k = k * (unsigned long long)a.nbk[m] + (unsigned long long)(a.idx[(i64)m * a.nnz + e] / a.bs[m]);
+  keys[e] = k;
+  pos[e] = (int32_t)e;
+}
+
+// local coordinates (index % bs) and values in the sorted order
+__global__ __launch_bounds__(256) void hicoo_gather_kernel(HiCooArgs a, const int32_t* __restrict__ perm,
+                                                           const float* __restrict__ vals, int32_t* __restrict__ local,
+                                                           float* __restrict__ vout) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= a.nnz) return;
+  const i64 src = perm[e];
+  for (int m = 0; m < a.nd; ++m) local[(i64)m * a.nnz + e] = (int32_t)(a.idx[(i64)m * a.nnz + src] % a.bs[m]);
+  vout[e] = vals[src];
+}
+
+int sp_export_hicoo(tenrso_ctx* ctx, const tenrso_sptensor* t, const i64* bs, std::vector<int64_t>& block_coords,
+                    std::vector<int64_t>& block_ptrs, std::vector<int32_t>& local, std::vector<float>& vals) {
+  const int nd = t->nd;
+  const i64 nnz = t->nnz;
+  block_coords.clear();
+  block_ptrs.assign(1, 0);
+  local.assign((size_t)nd * nnz, 0);
+  vals.assign((size_t)nnz, 0.f);
+  if (nnz == 0) return TENRSO_OK;
+  HiCooArgs ha{};
+  ha.nd = nd;
+  ha.nnz = nnz;
+  ha.idx = t->idx;
+  int bits = 0;
+  {
+    unsigned long long maxkey = 1;
+    for (int k = 0; k < nd; ++k) {
+      ha.bs[k] = bs[k];
+      ha.nbk[k] = (t->shape[k] + bs[k] - 1) / bs[k];
+      if (maxkey > ~0ull / (unsigned long long)ha.nbk[k]) {
+        set_error("HiCOO: the block grid has more than 2^64 blocks");
+        return TENRSO_ERR_UNSUPPORTED;
+      }
+      maxkey *= (unsigned long long)ha.nbk[k];
+    }
+    while (bits < 64 && (maxkey >> bits) != 0) ++bits;
+  }
+  unsigned long long *kin = nullptr, *kout = nullptr;
+  int32_t *pin = nullptr, *pout = nullptr, *dloc = nullptr;
+  float* dval = nullptr;
+  void* tmp = nullptr;
+  int rc = TENRSO_OK;
+  auto fail = [&](hipError_t e) {
+    set_error("HIP error in the HiCOO export: %s", hipGetErrorString(e));
+    rc = TENRSO_ERR_DEVICE;
+  };
+  hipError_t e = hipMalloc(&kin, (size_t)nnz * 8);
+  if (e == hipSuccess) e = hipMalloc(&kout, (size_t)nnz * 8);
+  if (e == hipSuccess) e = hipMalloc(&pin, (size_t)nnz * 4);
+  if (e == hipSuccess) e = hipMalloc(&pout, (size_t)nnz * 4);
+  if (e == hipSuccess) e = hipMalloc(&dloc, (size_t)nd * nnz * 4);
+  if (e == hipSuccess) e = hipMalloc(&dval, (size_t)nnz * 4);
+  size_t tmp_bytes = 0;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(hicoo_keys_kernel, dim3(cdiv(nnz, 256)), dim3(256), 0, ctx->stream, ha, kin, pin);
+    e = hipGetLastError();
+  }
+  // LSD radix sort: stable, so nonzeros of one block keep their COO order (the
+  // reference's sort_by is stable too)
+  if (e == hipSuccess)
+    e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, pin, pout, (size_t)nnz, 0, bits, ctx->stream);
+  if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16));
+  if (e == hipSuccess)
+    e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, pin, pout, (size_t)nnz, 0, bits, ctx->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(hicoo_gather_kernel, dim3(cdiv(nnz, 256)), dim3(256), 0, ctx->stream, ha, pout, t->vals, dloc,
+                       dval);
+    e = hipGetLastError();
+  }
+  std::vector<unsigned long long> keys;
+  if (e == hipSuccess) {
+    keys.resize((size_t)nnz);
+    e = hipMemcpyAsync(keys.data(), kout, (size_t)nnz * 8, hipMemcpyDeviceToHost, ctx->stream);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(local.data(), dloc, (size_t)nd * nnz * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(vals.data(), dval, (size_t)nnz * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) fail(e);
+  for (void* p : {(void*)kin, (void*)kout, (void*)pin, (void*)pout, (void*)dloc, (void*)dval, tmp})
+    if (p) (void)hipFree(p);
+  if (rc != TENRSO_OK) return rc;
+  // blocks: runs of equal keys; block coordinates decoded from the key
+  std::vector<unsigned long long> bkeys;
+  for (i64 i = 0; i < nnz; ++i)
+    if (i == 0 || keys[(size_t)i] != keys[(size_t)i - 1]) {
+      if (i > 0) block_ptrs.push_back(i);
+      bkeys.push_back(keys[(size_t)i]);
+    }
+  block_ptrs.push_back(nnz);
+  const size_t nb = bkeys.size();
+  block_coords.assign((size_t)nd * nb, 0);
+  for (size_t b = 0; b < nb; ++b) {
+    unsigned long long k = bkeys[b];
+    for (int m = nd - 1; m >= 0; --m) {
+      block_coords[(size_t)m * nb + b] = (int64_t)(k % (unsigned long long)ha.nbk[m]);
+      k /= (unsigned long long)ha.nbk[m];
+    }
+  }
+  return TENRSO_OK;
+}
+
 }  // namespace tenrso

@@ -36,5 +36,11 @@ void sp_destroy(tenrso_sptensor* t);
 int sp_mttkrp(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* F, i64 R, int mode, float* out);
 int sp_export_csf(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, std::vector<std::vector<int64_t>>& fptr,
                   std::vector<std::vector<int32_t>>& fids, std::vector<float>& vals);
+// HiCOO of the COO as uploaded (HiCooTensor::from_coo, hicoo.rs:121-205): block
+// coordinate = index / bs, local = index % bs, nonzeros stably sorted by block
+// coordinate (lexicographic), duplicates kept.  Outputs are host vectors:
+// block_coords (nd x blocks, SoA), block_ptrs (blocks + 1), local (nd x nnz, SoA).
+int sp_export_hicoo(tenrso_ctx* ctx, const tenrso_sptensor* t, const i64* bs, std::vector<int64_t>& block_coords,
+                    std::vector<int64_t>& block_ptrs, std::vector<int32_t>& local, std::vector<float>& vals);
 
 }  // namespace tenrso

@@ -25,7 +25,7 @@ import numpy as np
 from . import _lib
 from ._lib import CpOptions, CpResult, check, lib
 
-__all__ = ["CooError", "CooTensor", "CsfTensor", "SparseTensor"]
+__all__ = ["CooError", "CooTensor", "CsfTensor", "HiCooError", "HiCooTensor", "SparseTensor"]
 
 
 class CooError(ValueError):
@@ -193,6 +193,73 @@ class CsfTensor:
         return CooTensor(idx, self.vals, self._shape)
 
 
+class HiCooError(ValueError):
+    """HiCooError (hicoo.rs): an invalid block shape."""
+
+
+class HiCooTensor:
+    """HiCOO tensor (hicoo.rs:79-95), built on the device by SparseTensor.hicoo.
+
+    block_coords: (ndim, num_blocks) int64; block_ptrs: num_blocks + 1 (block b
+    holds nonzeros block_ptrs[b]..block_ptrs[b+1]); local_coords: (ndim, nnz)
+    int32; values: nnz float32.  Nonzeros are in the order of
+    HiCooTensor::from_coo (stable sort by block coordinate, duplicates kept)."""
+
+    def __init__(self, shape, block_shape, block_coords, block_ptrs, local_coords, values):
+        self._shape, self._block_shape = list(shape), list(block_shape)
+        self.block_coords_, self.block_ptrs_ = block_coords, block_ptrs
+        self.local_coords_, self.values_ = local_coords, values
+
+    def shape(self):
+        return list(self._shape)
+
+    def block_shape(self):
+        return list(self._block_shape)
+
+    def ndim(self):
+        return len(self._shape)
+
+    def nnz(self):
+        return int(self.values_.size)
+
+    def num_blocks(self):
+        return int(self.block_ptrs_.size) - 1
+
+    def block_coords(self):
+        return self.block_coords_
+
+    def block_ptrs(self):
+        return self.block_ptrs_
+
+    def local_coords(self):
+        return self.local_coords_
+
+    def values(self):
+        return self.values_
+
+    def density(self):
+        """hicoo.rs density: nnz over the number of elements of the shape."""
+        return self.nnz() / float(np.prod(self._shape))
+
+    def global_indices(self):
+        """(ndim, nnz) int64: block coordinate * block_shape + local coordinate."""
+        nb = self.num_blocks()
+        if nb == 0:
+            return np.zeros((self.ndim(), 0), np.int64)
+        owner = np.repeat(np.arange(nb), np.diff(self.block_ptrs_))
+        bs = np.asarray(self._block_shape, np.int64)[:, None]
+        return self.block_coords_[:, owner] * bs + self.local_coords_.astype(np.int64)
+
+    def to_coo(self):
+        """hicoo.rs:313-330: back to COO, in HiCOO order."""
+        if self.nnz() == 0:
+            return CooTensor.zeros(self._shape)
+        return CooTensor(self.global_indices(), self.values_, self._shape)
+
+    def to_dense(self):
+        return self.to_coo().to_dense()
+
+
 class SparseTensor:
     """A COO tensor uploaded to the GPU and sorted per output mode (tenrso_sp_create)."""
 
@@ -235,6 +302,27 @@ class SparseTensor:
         check(lib().tenrso_sp_mode_arrays(self.ctx.ptr, self.ptr, mode, rowptr.ctypes.data, row.ctypes.data,
                                           oth.ctypes.data, vals.ctypes.data))
         return rowptr, row, oth, vals
+
+    def hicoo(self, block_shape):
+        """HiCooTensor::from_coo (hicoo.rs:121-205) of the uploaded COO, on the device."""
+        nd = len(self.shape)
+        block_shape = [int(b) for b in block_shape]
+        if len(block_shape) != nd:
+            raise HiCooError(f"Invalid block shape: block_shape length {len(block_shape)} != ndim {nd}")
+        bsa = (C.c_int64 * nd)(*block_shape)
+        nb = C.c_int64(0)
+        rc = lib().tenrso_sp_hicoo(self.ctx.ptr, self.ptr, bsa, C.byref(nb), None, None, None, None)
+        if rc == _lib.ERR_INVALID_ARGUMENT:
+            raise HiCooError(_lib.last_error())
+        check(rc)
+        B = int(nb.value)
+        bc = np.zeros((nd, B), np.int64)
+        bp = np.zeros(B + 1, np.int64)
+        loc = np.zeros((nd, self.nnz), np.int32)
+        vals = np.zeros(self.nnz, np.float32)
+        check(lib().tenrso_sp_hicoo(self.ctx.ptr, self.ptr, bsa, C.byref(nb), bc.ctypes.data, bp.ctypes.data,
+                                    loc.ctypes.data, vals.ctypes.data))
+        return HiCooTensor(self.shape, block_shape, bc, bp, loc, vals)
 
     def csf(self, mode):
         """CsfTensor of mode order [mode, others ascending], built from the device sort."""

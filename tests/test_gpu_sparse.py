@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 import oracle  # noqa: E402
 from tenrso_amd import synth  # noqa: E402
-from tenrso_amd.sparse import CooError, CooTensor, CsfTensor, SparseTensor  # noqa: E402
+from tenrso_amd.sparse import CooError, CooTensor, CsfTensor, HiCooError, SparseTensor  # noqa: E402
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -201,3 +201,47 @@ def test_sparse_zipf_full_size_mttkrp():
         e = rel(sp.mttkrp(Fd, mode).cpu().numpy(), oracle.coo_mttkrp(idx, vals, shape, F64, mode))
         print(f"  mode {mode}: rel frob err {e:.3e}", flush=True)
         assert e <= 1e-5
+
+
+@pytest.mark.parametrize("shape,nnz,bs", [((7, 6, 5), 60, (2, 2, 2)), ((40, 30, 20), 5000, (8, 8, 8)),
+                                          ((9, 8, 7, 6), 700, (4, 3, 2, 5)), ((50, 60), 900, (16, 7)),
+                                          ((13, 9, 11), 400, (1, 1, 1)), ((13, 9, 11), 400, (13, 9, 11))])
+def test_device_hicoo_bit_exact(shape, nnz, bs):
+    """HiCooTensor::from_coo (hicoo.rs:121-205) on the device against the oracle's
+    restatement, duplicates included: block coordinates, block pointers, local
+    coordinates and values identical, and the round trip back to COO."""
+    print(f"[case] hicoo {shape} nnz {nnz} block {bs}", flush=True)
+    idx, vals = random_coo(shape, nnz, seed=11, dup=nnz // 10)
+    sp = SparseTensor(idx, vals, shape)
+    h = sp.hicoo(bs)
+    bc, bp, loc, v = oracle.hicoo_from_coo(idx, vals, shape, bs)
+    assert h.num_blocks() == bc.shape[1] and h.nnz() == idx.shape[1]
+    assert np.array_equal(h.block_coords(), bc)
+    assert np.array_equal(h.block_ptrs(), bp)
+    assert np.array_equal(h.local_coords(), loc)
+    assert np.array_equal(h.values(), v)
+    back = h.to_coo()
+    assert np.array_equal(back.indices(), np.asarray(bc[:, np.repeat(np.arange(bc.shape[1]), np.diff(bp))]
+                                                     * np.array(bs)[:, None] + loc))
+
+
+def test_device_hicoo_reference_cases_and_errors():
+    """hicoo.rs:414-575 through the device: 8^3 with blocks 4^3 gives two blocks
+    ((0,0,0) and (1,1,1)); density 2/1000; the 5-way case; an empty tensor has no
+    blocks; a block shape of the wrong length or with a zero is an error."""
+    sp = SparseTensor(np.array([[0, 1, 4], [1, 0, 5], [2, 3, 6]], np.int32), np.array([5.0, 6.0, 7.0], np.float32),
+                      (8, 8, 8))
+    h = sp.hicoo((4, 4, 4))
+    assert h.nnz() == 3 and h.num_blocks() == 2 and h.block_coords().T.tolist() == [[0, 0, 0], [1, 1, 1]]
+    d = SparseTensor(np.array([[0, 5], [0, 5], [0, 5]], np.int32), np.array([1.0, 2.0], np.float32), (10, 10, 10))
+    assert abs(d.hicoo((5, 5, 5)).density() - 0.002) < 1e-6
+    h5 = SparseTensor(np.array([[0, 2], [0, 2], [0, 2], [0, 2], [1, 2]], np.int32), np.array([1.0, 2.0], np.float32),
+                      (4, 4, 4, 4, 4)).hicoo((2,) * 5)
+    dense = h5.to_dense()
+    assert h5.ndim() == 5 and dense[0, 0, 0, 0, 1] == 1.0 and dense[2, 2, 2, 2, 2] == 2.0
+    e = SparseTensor(np.zeros((3, 0), np.int32), np.zeros(0, np.float32), (8, 8, 8)).hicoo((4, 4, 4))
+    assert e.nnz() == 0 and e.num_blocks() == 0 and e.block_ptrs().tolist() == [0]
+    with pytest.raises(HiCooError):
+        sp.hicoo((4, 4))
+    with pytest.raises(HiCooError):
+        sp.hicoo((4, 0, 4))

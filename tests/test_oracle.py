@@ -368,3 +368,51 @@ def test_svd_init_matches_eigenvectors_of_the_unfolding_gram(orc):
     assert all((g >= 0).all() for g in G)
     H = orc.svd_init(rng.standard_normal((3, 5, 4)), 4, "svd")
     assert np.isnan(H[0][:, 3]).all() and not np.isnan(H[0][:, :3]).any()
+
+
+def test_hicoo_from_coo_reference_cases(orc):
+    """hicoo.rs:414-575 (from_coo basic, empty, grouping, single element, 5-way,
+    invalid block shape) on the numpy restatement, plus the block and local
+    coordinates those cases imply."""
+    def build(shape, pts, bs):
+        idx = np.array([p for p, _ in pts], np.int64).T.reshape(len(shape), -1)
+        vals = np.array([v for _, v in pts], np.float64)
+        return orc.hicoo_from_coo(idx, vals, shape, bs)
+    bc, bp, loc, v = build((8, 8, 8), [((0, 1, 2), 5.0), ((1, 0, 3), 6.0), ((4, 5, 6), 7.0)], (4, 4, 4))
+    assert bp.tolist() == [0, 2, 3] and bc.T.tolist() == [[0, 0, 0], [1, 1, 1]]
+    assert loc.T.tolist() == [[0, 1, 2], [1, 0, 3], [0, 1, 2]] and v.tolist() == [5.0, 6.0, 7.0]
+    bc, bp, loc, v = build((8, 8, 8), [], (4, 4, 4))
+    assert bp.tolist() == [0] and bc.shape[1] == 0 and v.size == 0
+    bc, bp, _, _ = build((4, 4, 4), [((0, 0, 0), 1.0), ((0, 1, 1), 2.0), ((1, 0, 1), 3.0)], (2, 2, 2))
+    assert bp.tolist() == [0, 3]
+    bc, bp, _, _ = build((4, 4, 4), [((0, 0, 0), 1.0), ((2, 2, 2), 2.0)], (2, 2, 2))
+    assert bp.tolist() == [0, 1, 2]
+    bc, bp, loc, v = build((8, 8, 8), [((2, 3, 4), 42.0)], (4, 4, 4))
+    assert (bc[:, 0] * 4 + loc[:, 0]).tolist() == [2, 3, 4] and v.tolist() == [42.0]
+    bc, bp, loc, v = build((4, 4, 4, 4, 4), [((0, 0, 0, 0, 1), 1.0), ((2, 2, 2, 2, 2), 2.0)], (2,) * 5)
+    assert bp.tolist() == [0, 1, 2] and (bc * 2 + loc).T.tolist() == [[0, 0, 0, 0, 1], [2, 2, 2, 2, 2]]
+    for bad in [(4, 4), (4, 0, 4)]:
+        with pytest.raises(ValueError):
+            orc.hicoo_from_coo(np.zeros((3, 0), np.int64), np.zeros(0), (8, 8, 8), bad)
+
+
+def test_hicoo_from_coo_invariants(orc):
+    """Random COO with duplicates: block coordinates strictly increasing
+    (lexicographic), every nonzero inside its block, global = block * bs + local
+    reproduces the COO as a multiset, and within a block the COO order is kept
+    (stable sort)."""
+    rng = np.random.default_rng(3)
+    shape, bs = (13, 9, 11), (4, 2, 5)
+    idx = np.stack([rng.integers(0, n, 400) for n in shape])
+    idx = np.concatenate([idx, idx[:, :50]], axis=1)
+    vals = np.arange(idx.shape[1], dtype=np.float64)  # value = COO position
+    bc, bp, loc, v = orc.hicoo_from_coo(idx, vals, shape, bs)
+    keys = [tuple(c) for c in bc.T]
+    assert all(a < b for a, b in zip(keys, keys[1:]))
+    owner = np.repeat(np.arange(bc.shape[1]), np.diff(bp))
+    glob = bc[:, owner] * np.array(bs)[:, None] + loc
+    assert (loc < np.array(bs)[:, None]).all()
+    assert (glob == idx[:, v.astype(np.int64)]).all()
+    for b in range(bc.shape[1]):
+        seg = v[bp[b]:bp[b + 1]]
+        assert (np.diff(seg) > 0).all()
