@@ -101,7 +101,9 @@ def bf16_round(a):
     return (u.astype(np.uint32) << 16).view(np.float32)
 
 
-@pytest.mark.parametrize("shape,rank", [((128, 128, 64), 32), ((64, 96, 128), 64), ((32, 32, 32, 32), 16)])
+@pytest.mark.parametrize("shape,rank", [((128, 128, 64), 32), ((64, 96, 128), 64), ((32, 32, 32, 32), 16),
+                                        ((64, 128), 5), ((96, 64), 8), ((6, 5, 8, 7, 64), 3),
+                                        ((4, 5, 3, 6, 4, 64), 12)])
 def test_mttkrp_bf16_error_bound(shape, rank):
     print(f"[case] bf16 mttkrp {shape} rank {rank}", flush=True)
     rng = np.random.default_rng(1)
