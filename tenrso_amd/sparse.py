@@ -250,6 +250,12 @@ class HiCooTensor:
         bs = np.asarray(self._block_shape, np.int64)[:, None]
         return self.block_coords_[:, owner] * bs + self.local_coords_.astype(np.int64)
 
+    def iter(self):
+        """HiCooIterator (hicoo.rs:290-292): (global indices, value) in HiCOO order."""
+        g = self.global_indices()
+        for e in range(self.nnz()):
+            yield [int(v) for v in g[:, e]], float(self.values_[e])
+
     def to_coo(self):
         """hicoo.rs:313-330: back to COO, in HiCOO order."""
         if self.nnz() == 0:

@@ -235,6 +235,9 @@ def test_device_hicoo_reference_cases_and_errors():
     assert h.nnz() == 3 and h.num_blocks() == 2 and h.block_coords().T.tolist() == [[0, 0, 0], [1, 1, 1]]
     d = SparseTensor(np.array([[0, 5], [0, 5], [0, 5]], np.int32), np.array([1.0, 2.0], np.float32), (10, 10, 10))
     assert abs(d.hicoo((5, 5, 5)).density() - 0.002) < 1e-6
+    it = SparseTensor(np.array([[0, 2], [0, 2], [1, 2]], np.int32), np.array([1.0, 2.0], np.float32),
+                      (4, 4, 4)).hicoo((2, 2, 2))
+    assert list(it.iter()) == [([0, 0, 1], 1.0), ([2, 2, 2], 2.0)]  # hicoo.rs:451-462
     h5 = SparseTensor(np.array([[0, 2], [0, 2], [0, 2], [0, 2], [1, 2]], np.int32), np.array([1.0, 2.0], np.float32),
                       (4, 4, 4, 4, 4)).hicoo((2,) * 5)
     dense = h5.to_dense()
