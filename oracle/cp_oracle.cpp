@@ -5,7 +5,7 @@
 // checker.  Nothing in the product (tenrso_amd/) links or calls this file.
 //
 // Every function cites the reference file:line it restates.  Two instantiations:
-//   * T = double : the parity oracle (all correctness bars are judged against it)
+//   * T = double : the parity oracle (every correctness bar is checked against it)
 //   * T = float  : "reference as written" (sequential f32 sums, no FMA), reported
 //                  beside the oracle for information.
 // Build: g++ -O2 -ffp-contract=off -fopenmp -shared -fPIC (oracle/build_oracle.py).
