@@ -111,7 +111,9 @@ int tenrso_convert_f32_to_bf16(tenrso_ctx* ctx, const float* src, int64_t n, uin
 /* Profiling aid (no reference counterpart): launches only the main MFMA (or
  * fallback) MTTKRP kernel `reps` times back to back between two HIP events on
  * the context's stream and returns the average device time per launch in *ms.
- * precision selects the fp32 (X: float*) or bf16 (X: uint16_t*) kernel. */
+ * precision selects the fp32 (X: float*) or bf16 (X: uint16_t*) kernel, or, with
+ * 2, the three-way bf16 split kernel that the BF16 cp_als runs for its last mode
+ * (X: float*, mode = ndim - 1). */
 int tenrso_mttkrp_profile(tenrso_ctx* ctx, int precision, const void* X, int ndim, const int64_t* shape,
                           const float* const* factors, int64_t rank, int mode, int reps, double* ms);
 

@@ -14,15 +14,23 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
                                        linearity, faithful == fast == blocked-parallel.
   khatri_rao                           pinned: khatri_rao.rs doc test + tests 479-554.
   gram / gram_hadamard                 pinned: cp/tests.rs:26-36 KAT, symmetry, einsum.
-  solve_ls                             pinned by the normal equations (A G = M) and
-                                       numpy.linalg.lstsq; scirs2_linalg::lstsq itself
-                                       is third-party and absent -> its rounding and
-                                       failure condition are parity unpinned.
+  solve_ls                             pinned by the normal equations (A G = M),
+                                       numpy.linalg.lstsq, the closed form of the ridge
+                                       retry on a singular diagonal system (for the f32
+                                       and f64 epsilons) and LinalgError on systems that
+                                       fail the retry; scirs2_linalg::lstsq itself is
+                                       third-party and absent -> its rounding and its
+                                       failure condition on borderline systems are
+                                       parity unpinned.
   fit / recon_norm_sq / inner product  pinned: explicit dense 1-||X-Xhat||/||X||.
   cp_als (f64, f32, constrained)       pinned: exact low-rank recovery (fit -> 1),
                                        normal-equation residual of every update,
-                                       convergence-rule unit cases.  orthogonal=True
-                                       uses Householder QR: parity unpinned.
+                                       convergence-rule unit cases, the oscillation
+                                       rule against the fit history, LinalgError
+                                       propagation.  x_lead (leading modes on another
+                                       tensor) is pinned by x_lead = X reproducing the
+                                       plain run bit for bit.  orthogonal=True uses
+                                       Householder QR: parity unpinned.
   cp_reconstruct / extract_weights     pinned: numpy einsum, KAT outer.rs doc test.
   cp_factor_grads                      pinned: FP64 central differences, identity
                                        dL/dA_n = A_n G_n - M_n.

@@ -13,9 +13,10 @@
 //
 // Parity-unpinned pieces (no reference test fixes them; see the module docs in
 // oracle/oracle.py): the per-row `scirs2_linalg::lstsq` solve is restated as a
-// Cholesky solve of the SPD normal matrix with the reference's ridge retry and a
-// symmetric-eigen pseudo-inverse as last resort; `orthonormalize_factor` uses
-// Householder QR (scirs2_linalg::qr's algorithm is not in the reference).
+// Cholesky solve of the SPD normal matrix with the reference's ridge retry, and a
+// system that fails both factorisations is a LinalgError, as in the reference;
+// `orthonormalize_factor` uses Householder QR (scirs2_linalg::qr's algorithm is
+// not in the reference).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -268,67 +269,16 @@ void chol_solve(const std::vector<T>& Lc, i64 n, T* b) {
   }
 }
 
-// Symmetric eigen-decomposition (cyclic Jacobi) -> minimum-norm pseudo-inverse.
-// Last resort when the (ridge-regularised) normal matrix is not positive definite.
-template <class T>
-std::vector<T> pinv_sym(const T* Ain, i64 n) {
-  std::vector<double> A((size_t)(n * n)), V((size_t)(n * n), 0.0);
-  for (i64 x = 0; x < n * n; ++x) A[(size_t)x] = (double)Ain[x];
-  for (i64 i = 0; i < n; ++i) V[(size_t)(i * n + i)] = 1.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0.0;
-    for (i64 p = 0; p < n; ++p)
-      for (i64 q = p + 1; q < n; ++q) off += A[(size_t)(p * n + q)] * A[(size_t)(p * n + q)];
-    if (off < 1e-30) break;
-    for (i64 p = 0; p < n; ++p)
-      for (i64 q = p + 1; q < n; ++q) {
-        double apq = A[(size_t)(p * n + q)];
-        if (std::fabs(apq) < 1e-300) continue;
-        double app = A[(size_t)(p * n + p)], aqq = A[(size_t)(q * n + q)];
-        double theta = (aqq - app) / (2.0 * apq);
-        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        for (i64 k = 0; k < n; ++k) {
-          double akp = A[(size_t)(k * n + p)], akq = A[(size_t)(k * n + q)];
-          A[(size_t)(k * n + p)] = c * akp - s * akq;
-          A[(size_t)(k * n + q)] = s * akp + c * akq;
-        }
-        for (i64 k = 0; k < n; ++k) {
-          double apk = A[(size_t)(p * n + k)], aqk = A[(size_t)(q * n + k)];
-          A[(size_t)(p * n + k)] = c * apk - s * aqk;
-          A[(size_t)(q * n + k)] = s * apk + c * aqk;
-        }
-        for (i64 k = 0; k < n; ++k) {
-          double vkp = V[(size_t)(k * n + p)], vkq = V[(size_t)(k * n + q)];
-          V[(size_t)(k * n + p)] = c * vkp - s * vkq;
-          V[(size_t)(k * n + q)] = s * vkp + c * vkq;
-        }
-      }
-  }
-  double emax = 0.0;
-  for (i64 i = 0; i < n; ++i) emax = std::max(emax, std::fabs(A[(size_t)(i * n + i)]));
-  const double cut = emax * (double)n * std::numeric_limits<double>::epsilon();
-  std::vector<T> P((size_t)(n * n), T(0));
-  for (i64 i = 0; i < n; ++i)
-    for (i64 j = 0; j < n; ++j) {
-      double s = 0.0;
-      for (i64 k = 0; k < n; ++k) {
-        double e = A[(size_t)(k * n + k)];
-        if (std::fabs(e) > cut) s += V[(size_t)(i * n + k)] * V[(size_t)(j * n + k)] / e;
-      }
-      P[(size_t)(i * n + j)] = (T)s;
-    }
-  return P;
-}
-
 // solve_least_squares (cp/helpers.rs:455-495): for every row i, x = lstsq(G^T, M[i,:]).
 // G^T is the symmetric normal matrix; lstsq's exact solution for SPD G^T is the
 // Cholesky solution (restated here).  On failure the reference adds
-// T::epsilon()*10R to the diagonal and retries (helpers.rs:477-490); we do the
-// same, then fall back to the eigen pseudo-inverse.  Returns the path taken:
-// 0 = Cholesky, 1 = ridge retry, 2 = pseudo-inverse.
+// T::epsilon()*10R to the diagonal and retries (helpers.rs:477-490) and returns
+// CpError::LinalgError if that fails too (helpers.rs:485).  ridge_eps is the
+// epsilon of the instantiation being restated (<= 0: the epsilon of T).
+// Returns the path taken: 0 = Cholesky, 1 = ridge retry, -1 = LinalgError (out
+// untouched).
 template <class T>
-int solve_ls(const T* M, i64 rows, i64 R, const T* G, T* out) {
+int solve_ls(const T* M, i64 rows, i64 R, const T* G, T* out, double ridge_eps = 0.0) {
   std::vector<T> Gt((size_t)(R * R));
   for (i64 i = 0; i < R; ++i)
     for (i64 j = 0; j < R; ++j) Gt[(size_t)(i * R + j)] = G[j * R + i];
@@ -337,19 +287,10 @@ int solve_ls(const T* M, i64 rows, i64 R, const T* G, T* out) {
   if (!cholesky(Lc, R)) {
     path = 1;
     Lc = Gt;
-    const T eps = std::numeric_limits<T>::epsilon() * (T)(R * 10);
+    const T e = ridge_eps > 0.0 ? (T)ridge_eps : std::numeric_limits<T>::epsilon();
+    const T eps = e * (T)(R * 10);
     for (i64 k = 0; k < R; ++k) Lc[(size_t)(k * R + k)] += eps;
-    if (!cholesky(Lc, R)) path = 2;
-  }
-  if (path == 2) {
-    std::vector<T> P = pinv_sym(Gt.data(), R);
-    for (i64 i = 0; i < rows; ++i)
-      for (i64 j = 0; j < R; ++j) {
-        T s = T(0);
-        for (i64 k = 0; k < R; ++k) s += P[(size_t)(j * R + k)] * M[i * R + k];
-        out[i * R + j] = s;
-      }
-    return path;
+    if (!cholesky(Lc, R)) return -1;
   }
   for (i64 i = 0; i < rows; ++i) {
     for (i64 j = 0; j < R; ++j) out[i * R + j] = M[i * R + j];
@@ -532,7 +473,7 @@ struct orc_result {
   int oscillation_count;
   double fit;
   double final_fit_change;
-  int solve_path_max;   // worst solve_ls path taken (0/1/2)
+  int solve_path_max;   // worst solve_ls path taken (0 Cholesky, 1 ridge retry)
   double seconds;       // wall time of the ALS loop
 };
 
@@ -575,8 +516,8 @@ void orc_gram_f64(const double* Fm, int64_t rows, int64_t R, double* G) { gram(F
 void orc_gram_hadamard_f64(const double* const* F, int nd, const int64_t* shape, int64_t R, int skip, double* out) {
   gram_hadamard(F, nd, shape, R, skip, out);
 }
-int orc_solve_ls_f64(const double* M, int64_t rows, int64_t R, const double* G, double* out) {
-  return solve_ls(M, rows, R, G, out);
+int orc_solve_ls_f64(const double* M, int64_t rows, int64_t R, const double* G, double* out, double ridge_eps) {
+  return solve_ls(M, rows, R, G, out, ridge_eps);
 }
 void orc_tikhonov_f64(double* G, int64_t n, double lam, int64_t order) { tikhonov(G, n, lam, order); }
 void orc_soft_threshold_f64(double* Fm, int64_t n, double th) { soft_threshold(Fm, n, th); }
@@ -594,11 +535,16 @@ double orc_fit_f64(const double* X, int nd, const int64_t* shape, const double* 
 // cp_als (cp/core.rs:59-182) and cp_als_constrained (cp/core.rs:225-408) with the
 // initial factors supplied by the caller (the reference draws them from an
 // unseeded RNG, cp/helpers.rs:212-231).  `fast` selects the cache-friendly FP64
-// MTTKRP (only for T = double).  time_limit < 0 means None.
+// MTTKRP (only for T = double).  time_limit < 0 means None.  ridge_eps: see
+// solve_ls.  Xlead (may be null): the tensor the MTTKRPs of modes 0..N-2
+// contract instead of X (the model of an ALS whose leading modes read a rounded
+// copy of X while the last mode, the norm and the fit use X itself).
+// Returns 13 (LinalgError) when a solve fails after its ridge retry.
 }  // extern "C"
 template <class T>
 int cp_als_impl(const T* X, int nd, const int64_t* shape, int64_t R, T** F, int max_iters, double tol,
-                double time_limit, const orc_constraints* c, int fast, double* fit_history, orc_result* res) {
+                double time_limit, const orc_constraints* c, int fast, double* fit_history, orc_result* res,
+                double ridge_eps = 0.0, const T* Xlead = nullptr) {
   if (R == 0) return 10;
   for (int k = 0; k < nd; ++k)
     if (R > shape[k]) return 10;                       // InvalidRank (core.rs:82-91)
@@ -626,17 +572,20 @@ int cp_als_impl(const T* X, int nd, const int64_t* shape, int64_t R, T** F, int 
     iters = iter + 1;
     for (int mode = 0; mode < nd; ++mode) {
       M.assign((size_t)(shape[mode] * R), T(0));
+      const T* Xm = (Xlead && mode < nd - 1) ? Xlead : X;
       if constexpr (std::is_same<T, double>::value) {
-        if (fast) mttkrp_fast_f64(X, nd, shape, F, R, mode, M.data());
-        else mttkrp_faithful(X, nd, shape, (const T* const*)F, R, mode, M.data());
+        if (fast) mttkrp_fast_f64(Xm, nd, shape, F, R, mode, M.data());
+        else mttkrp_faithful(Xm, nd, shape, (const T* const*)F, R, mode, M.data());
       } else {
-        mttkrp_faithful(X, nd, shape, (const T* const*)F, R, mode, M.data());
+        mttkrp_faithful(Xm, nd, shape, (const T* const*)F, R, mode, M.data());
       }
       gram_hadamard((const T* const*)F, nd, shape, R, mode, G.data());
       if (cc.tikhonov_order > 0) tikhonov(G.data(), R, (T)cc.tikhonov_lambda, cc.tikhonov_order);
       else if (cc.l2 > 0.0)
         for (i64 i = 0; i < R; ++i) G[(size_t)(i * R + i)] += (T)cc.l2;
-      worst = std::max(worst, solve_ls(M.data(), shape[mode], R, G.data(), F[mode]));
+      const int path = solve_ls(M.data(), shape[mode], R, G.data(), F[mode], ridge_eps);
+      if (path < 0) return 13;  // CpError::LinalgError (helpers.rs:485)
+      worst = std::max(worst, path);
       if (cc.l1 > 0.0) soft_threshold(F[mode], shape[mode] * R, (T)cc.l1);
       if (cc.nonnegative)
         for (i64 x = 0; x < shape[mode] * R; ++x) F[mode][x] = std::max(F[mode][x], T(0));
@@ -666,12 +615,13 @@ int cp_als_impl(const T* X, int nd, const int64_t* shape, int64_t R, T** F, int 
 
 extern "C" {
 int orc_cp_als_f64(const double* X, int nd, const int64_t* shape, int64_t R, double** F, int max_iters, double tol,
-                   double time_limit, const orc_constraints* c, int fast, double* hist, orc_result* res) {
-  return cp_als_impl(X, nd, shape, R, F, max_iters, tol, time_limit, c, fast, hist, res);
+                   double time_limit, const orc_constraints* c, int fast, double* hist, orc_result* res,
+                   double ridge_eps, const double* Xlead) {
+  return cp_als_impl(X, nd, shape, R, F, max_iters, tol, time_limit, c, fast, hist, res, ridge_eps, Xlead);
 }
 int orc_cp_als_f32(const float* X, int nd, const int64_t* shape, int64_t R, float** F, int max_iters, double tol,
-                   double time_limit, const orc_constraints* c, double* hist, orc_result* res) {
-  return cp_als_impl(X, nd, shape, R, F, max_iters, tol, time_limit, c, 0, hist, res);
+                   double time_limit, const orc_constraints* c, double* hist, orc_result* res, double ridge_eps) {
+  return cp_als_impl(X, nd, shape, R, F, max_iters, tol, time_limit, c, 0, hist, res, ridge_eps);
 }
 
 // cp_reconstruct (tenrso-kernels/src/outer.rs:223-274 with outer_product 109-136,

@@ -12,7 +12,7 @@ __all__ = [
     "solve_ls", "tikhonov", "soft_threshold", "orthonormalize", "norm_sq", "recon_norm_sq", "fit",
     "cp_als", "cp_reconstruct", "extract_weights", "cp_factor_grads", "coo_mttkrp", "num_threads", "svd_init",
     "hicoo_from_coo",
-    "OrcConstraints", "REASONS",
+    "OrcConstraints", "REASONS", "LinalgError", "F32_EPS",
 ]
 
 REASONS = ["FitTolerance", "MaxIterations", "Oscillation", "TimeLimit"]
@@ -126,12 +126,21 @@ def gram_hadamard(factors, skip):
     return out
 
 
-def solve_ls(M, G):
-    """solve_least_squares (cp/helpers.rs:455-495); returns (X, path)."""
+class LinalgError(ValueError):
+    """CpError::LinalgError (cp/types.rs:13-35): a solve failed after the ridge retry."""
+
+
+def solve_ls(M, G, ridge_eps=None):
+    """solve_least_squares (cp/helpers.rs:455-495); returns (X, path), path 0 for the
+    plain solve and 1 for the ridge retry (Gamma + eps 10R I, eps = ridge_eps or the
+    FP64 epsilon).  Raises LinalgError when the retry fails too (helpers.rs:485)."""
     M = np.ascontiguousarray(M, dtype=np.float64)
     G = np.ascontiguousarray(G, dtype=np.float64)
     out = np.zeros_like(M)
-    path = lib().orc_solve_ls_f64(_p(M), C.c_int64(M.shape[0]), C.c_int64(M.shape[1]), _p(G), _p(out))
+    path = lib().orc_solve_ls_f64(_p(M), C.c_int64(M.shape[0]), C.c_int64(M.shape[1]), _p(G), _p(out),
+                                  C.c_double(0.0 if ridge_eps is None else float(ridge_eps)))
+    if path < 0:
+        raise LinalgError("Linear algebra error: lstsq failed after the ridge retry")
     return out, path
 
 
@@ -195,11 +204,22 @@ def fit(X, factors, tensor_norm_sq=None, fast=True):
                                    C.c_double(tensor_norm_sq), int(fast)))
 
 
+F32_EPS = float(np.finfo(np.float32).eps)
+
+
 def cp_als(X, init_factors, max_iters, tol, time_limit=None, nonnegative=False, orthogonal=False, l2=0.0,
-           l1=0.0, tikhonov_order=0, tikhonov_lambda=0.0, dtype=np.float64, fast=True):
+           l1=0.0, tikhonov_order=0, tikhonov_lambda=0.0, dtype=np.float64, fast=True, ridge_eps=F32_EPS,
+           x_lead=None):
     """cp_als / cp_als_constrained (cp/core.rs:59-182, 225-408) from explicit initial factors.
 
+    ridge_eps: the T::epsilon() of the ridge retry (helpers.rs:477-490).  It
+    defaults to f32's, the epsilon of the T = f32 instantiation that the device
+    library replaces; None takes the epsilon of `dtype`.
+    x_lead: optional tensor that the MTTKRPs of modes 0..N-2 contract instead of
+    X (FP64 model of the BF16 ALS: leading modes on bf16(X), the last mode, the
+    norm and the fit on X itself).
     Returns (factors, info) where info mirrors CpDecomp + ConvergenceInfo.
+    Raises LinalgError when a solve fails after its ridge retry.
     """
     X, F = _prep(X, init_factors, dtype)
     F = [f.copy() for f in F]
@@ -209,13 +229,23 @@ def cp_als(X, init_factors, max_iters, tol, time_limit=None, nonnegative=False, 
     res = OrcResult()
     hist = np.zeros(max(max_iters, 1), dtype=np.float64)
     tl = -1.0 if time_limit is None else float(time_limit)
+    eps = C.c_double(0.0 if ridge_eps is None else float(ridge_eps))
     if dtype == np.float64:
+        lead = None
+        if x_lead is not None:
+            lead = np.ascontiguousarray(x_lead, dtype=np.float64)
+            if lead.shape != X.shape:
+                raise ValueError("x_lead must have the shape of X")
         rc = lib().orc_cp_als_f64(_p(X), X.ndim, _shape(X.shape), C.c_int64(R), _ptrs(F), max_iters,
                                   C.c_double(tol), C.c_double(tl), C.byref(cons), int(fast), _p(hist),
-                                  C.byref(res))
+                                  C.byref(res), eps, None if lead is None else _p(lead))
     else:
+        if x_lead is not None:
+            raise ValueError("x_lead is supported by the FP64 oracle only")
         rc = lib().orc_cp_als_f32(_p(X), X.ndim, _shape(X.shape), C.c_int64(R), _ptrs(F), max_iters,
-                                  C.c_double(tol), C.c_double(tl), C.byref(cons), _p(hist), C.byref(res))
+                                  C.c_double(tol), C.c_double(tl), C.byref(cons), _p(hist), C.byref(res), eps)
+    if rc == 13:
+        raise LinalgError("Linear algebra error: lstsq failed after the ridge retry")
     if rc:
         raise ValueError({10: "InvalidRank", 11: "InvalidTolerance", 12: "NonnegativeViolation"}.get(rc, rc))
     info = dict(iters=res.iters, reason=REASONS[res.reason], oscillation_count=res.oscillation_count,

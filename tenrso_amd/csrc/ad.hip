@@ -24,58 +24,89 @@ namespace tenrso {
 
 static inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); }
 
-// Gamma = hadamard_{k != skip} G_k (FP64, R x R)
-__global__ __launch_bounds__(256) void hadamard_gram_kernel(const double* __restrict__ G, int nd, int skip, int R,
-                                                            double* __restrict__ Gamma) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= R * R) return;
-  double v = 1.0;
-  for (int k = 0; k < nd; ++k)
-    if (k != skip) v = v * G[(i64)k * R * R + e];
-  Gamma[e] = v;
-}
-
-// grad[i, c] = sum_r A[i, r] Gamma[r, c] - M[i, c]   (one workgroup per row, FP64)
-__global__ __launch_bounds__(64) void grad_rows_kernel(const float* __restrict__ A, const double* __restrict__ Gamma,
-                                                       const double* __restrict__ M, i64 rows, int R,
-                                                       float* __restrict__ grad) {
-  __shared__ double sa[64];
-  const i64 i = blockIdx.x;
-  const int c = threadIdx.x;
-  for (int r = c; r < R; r += 64) sa[r] = (double)A[i * R + r];
-  __syncthreads();
-  for (int cc = c; cc < R; cc += 64) {
-    double s0 = 0.0, s1 = 0.0;
-    int r = 0;
-    for (; r + 2 <= R; r += 2) {
-      s0 += sa[r] * Gamma[r * R + cc];
-      s1 += sa[r + 1] * Gamma[(r + 1) * R + cc];
+// One workgroup per row i of mode n (256 threads = 4 quarters x 64 columns):
+//   M[i, c]    = FP64 sum of the MTTKRP split partials of (i, c), each quarter of
+//                the splits summed by one thread group, quarters added in order;
+//   grad[i, c] = sum_r A[i, r] Gamma_n[r, c] - M[i, c], Gamma_n = hadamard_{k != n}
+//                G_k formed on the fly (each thread its quarter of column c, loaded
+//                beside the partials), r split over the same quarters;
+//   ip_part[i] = sum_c M[i, c] A[i, c]  (the loss's <X, Xhat> for the last mode).
+// grad and ip_part may be null.  This replaces a reduction launch, a Gamma launch
+// and a dot-product launch per mode; every sum has a fixed order.
+constexpr int GR_NE = 64;  // rank bound (SOLVE_RMAX)
+__global__ __launch_bounds__(256) void grad_rows_partials_kernel(const float* __restrict__ part, int splits,
+                                                                 i64 rows, int R, const double* __restrict__ G,
+                                                                 int nd, int n, const float* __restrict__ A,
+                                                                 float* __restrict__ grad,
+                                                                 double* __restrict__ ip_part) {
+  __shared__ double sq[4][GR_NE];
+  __shared__ double sM[GR_NE];
+  __shared__ double sA[GR_NE];
+  const int c = threadIdx.x & 63, qd = threadIdx.x >> 6;
+  const i64 row = blockIdx.x;
+  const bool colok = c < R;
+  const int r0 = qd * R / 4, r1 = (qd + 1) * R / 4;
+  double w[GR_NE / 4];
+  if (grad) {
+#pragma unroll
+    for (int k = 0; k < GR_NE / 4; ++k) {
+      double v = 0.0;
+      if (colok && r0 + k < r1) {
+        v = 1.0;
+        for (int m = 0; m < nd; ++m)
+          if (m != n) v = v * G[((i64)m * R + r0 + k) * R + c];
+      }
+      w[k] = v;
     }
-    if (r < R) s0 += sa[r] * Gamma[r * R + cc];
-    grad[i * R + cc] = (float)((s0 + s1) - M[i * R + cc]);
   }
-}
-
-// partial <M, A> per row block (64 rows), FP64
-__global__ __launch_bounds__(256) void dot_rows_kernel(const double* __restrict__ M, const float* __restrict__ A,
-                                                       i64 n, double* __restrict__ part) {
-  __shared__ double red[256];
-  const i64 b0 = (i64)blockIdx.x * 4096;
-  double s = 0.0;
-  for (i64 e = b0 + threadIdx.x; e < n && e < b0 + 4096; e += 256) s += M[e] * (double)A[e];
-  red[threadIdx.x] = s;
+  double m = 0.0;
+  if (colok) {
+    const i64 nn = rows * R, off = row * R + c;
+    const int s0 = qd * splits / 4, s1 = (qd + 1) * splits / 4;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int sp = s0;
+    for (; sp + 4 <= s1; sp += 4) {
+      a0 += (double)part[(i64)(sp + 0) * nn + off];
+      a1 += (double)part[(i64)(sp + 1) * nn + off];
+      a2 += (double)part[(i64)(sp + 2) * nn + off];
+      a3 += (double)part[(i64)(sp + 3) * nn + off];
+    }
+    for (; sp < s1; ++sp) a0 += (double)part[(i64)sp * nn + off];
+    m = (a0 + a1) + (a2 + a3);
+    if (qd == 0) sA[c] = (double)A[row * R + c];
+  }
+  sq[qd][c] = m;
   __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+  if (qd == 0) sM[c] = (sq[0][c] + sq[1][c]) + (sq[2][c] + sq[3][c]);
+  __syncthreads();
+  if (grad) {
+    double x0 = 0.0, x1 = 0.0;
+    if (colok) {
+#pragma unroll
+      for (int k = 0; k < GR_NE / 4; k += 2) {
+        if (r0 + k < r1) x0 += sA[r0 + k] * w[k];
+        if (r0 + k + 1 < r1) x1 += sA[r0 + k + 1] * w[k + 1];
+      }
+    }
+    sq[qd][c] = x0 + x1;
     __syncthreads();
+    if (qd == 0 && colok)
+      grad[row * R + c] = (float)(((sq[0][c] + sq[1][c]) + (sq[2][c] + sq[3][c])) - sM[c]);
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  if (ip_part && qd == 0) {
+    double ip = colok ? sM[c] * sA[c] : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ip += __shfl_xor(ip, o, 64);
+    if (c == 0) ip_part[row] = ip;
+  }
 }
 
-// loss = 0.5 (||X||^2 - 2 sum(part) + sum_rs prod_k G_k[r,s]); also ||grad||_F scale for lr
+// loss = 0.5 (||X||^2 - 2 sum(part) + sum_rs prod_k G_k[r,s]) into out[0] and, when
+// hist is not null, into *hist (the device-side loss history of a descent)
 __global__ __launch_bounds__(256) void loss_kernel(const double* __restrict__ G, int nd, int R,
                                                    const double* __restrict__ part, int npart,
-                                                   const double* __restrict__ scal, double* __restrict__ out) {
+                                                   const double* __restrict__ scal, double* __restrict__ out,
+                                                   double* __restrict__ hist) {
   __shared__ double red[256];
   double rec = 0.0;
   for (int e = threadIdx.x; e < R * R; e += 256) {
@@ -100,9 +131,11 @@ __global__ __launch_bounds__(256) void loss_kernel(const double* __restrict__ G,
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = 0.5 * (scal[SC_NORM_X] - 2.0 * red[0] + recon);
+    const double loss = 0.5 * (scal[SC_NORM_X] - 2.0 * red[0] + recon);
+    out[0] = loss;
     out[1] = red[0];
     out[2] = recon;
+    if (hist) *hist = loss;
   }
 }
 
@@ -129,18 +162,28 @@ __global__ __launch_bounds__(256) void gamma_norms_kernel(const double* __restri
   }
 }
 
-// A -= step * grad, step = lr (lr > 0) or 1 / max_n ||Gamma_n||_F read from gnorm
-__global__ __launch_bounds__(256) void axpy_step_kernel(float* __restrict__ A, const float* __restrict__ g, i64 n,
-                                                        double lr, const double* __restrict__ gnorm, int nd) {
-  const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+// A_k -= step * grad_k for every factor in one launch (the factors' elements are
+// laid end to end: element e belongs to the factor k with off[k] <= e < off[k+1]),
+// step = lr (lr > 0) or 1 / max_n ||Gamma_n||_F read from gnorm
+struct AxpyMany {
+  float* A[MAXD];
+  const float* g[MAXD];
+  i64 off[MAXD + 1];
+};
+__global__ __launch_bounds__(256) void axpy_step_kernel(AxpyMany a, int nd, double lr,
+                                                        const double* __restrict__ gnorm) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= a.off[nd]) return;
+  int k = 0;
+  while (e >= a.off[k + 1]) ++k;
+  const i64 i = e - a.off[k];
   double step = lr;
   if (lr <= 0.0) {
     double mx = 0.0;
-    for (int k = 0; k < nd; ++k) mx = fmax(mx, gnorm[k]);
+    for (int m = 0; m < nd; ++m) mx = fmax(mx, gnorm[m]);
     step = mx > 0.0 ? 1.0 / mx : 0.0;
   }
-  A[i] = (float)((double)A[i] - step * (double)g[i]);
+  a.A[k][i] = (float)((double)a.A[k][i] - step * (double)a.g[k][i]);
 }
 
 __global__ __launch_bounds__(256) void scale_cols_kernel(float* __restrict__ A, i64 rows, i64 R,
@@ -195,7 +238,8 @@ int loss_work(tenrso_ctx* ctx, int nd, const i64* shape, i64 R, LossWork* w) {
   TENRSO_TRY(ctx_buffer(ctx, SLOT_GRAMS, (size_t)(nd + 1) * R * R * sizeof(double), &pG));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_SOLVE, (size_t)std::max<i64>(R * R, 4096) * sizeof(double), &pGa));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_M64, (size_t)maxI * R * sizeof(double), &pM));
-  TENRSO_TRY(ctx_buffer(ctx, SLOT_IP, (size_t)std::max<i64>(4096, maxI * R / 4096 + 16) * sizeof(double), &pp));
+  // <M, A> partials (one per row of the last mode) and the norm's partials
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_IP, (size_t)std::max<i64>(4096, maxI + 16) * sizeof(double), &pp));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_SCALARS, 64 * sizeof(double), &ps));
   w->G = (double*)pG;
   w->Gamma = (double*)pGa;
@@ -207,27 +251,27 @@ int loss_work(tenrso_ctx* ctx, int nd, const i64* shape, i64 R, LossWork* w) {
 }
 }  // namespace
 
-// grads[n] = A_n Gamma_n - mttkrp(X, A, n); loss into w.out[0] (device), with ||X||^2
-// already in w.scal[SC_NORM_X].
+// grads[n] = A_n Gamma_n - mttkrp(X, A, n); loss into w.out[0] (device, and into
+// *hist when not null), with ||X||^2 already in w.scal[SC_NORM_X].  Per mode: the
+// MTTKRP as split partials, then one fused row kernel; modes without a gradient
+// to form are skipped except the last, whose MTTKRP gives <X, Xhat>.
 static int loss_grad_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, const float* const* F, i64 R,
-                            float* const* grads, LossWork& w) {
-  for (int k = 0; k < nd; ++k) TENRSO_TRY(gram_f64(ctx, F[k], shape[k], R, w.G + (i64)k * R * R));
-  int npart = 1;
+                            float* const* grads, LossWork& w, double* hist) {
+  TENRSO_TRY(gram_f64_many(ctx, nd, F, shape, R, w.G));
   for (int n = 0; n < nd; ++n) {
-    TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, X, nullptr, nd, shape, F, R, n, MttkrpOut::F64, w.M, nullptr));
-    if (n == nd - 1) {
-      npart = (int)cdiv(shape[n] * R, 4096);
-      hipLaunchKernelGGL(dot_rows_kernel, dim3(npart), dim3(256), 0, ctx->stream, w.M, F[n], shape[n] * R, w.part);
-    }
-    if (grads && grads[n]) {
-      hipLaunchKernelGGL(hadamard_gram_kernel, dim3(cdiv(R * R, 256)), dim3(256), 0, ctx->stream, w.G, nd, n,
-                         (int)R, w.Gamma);
-      hipLaunchKernelGGL(grad_rows_kernel, dim3((unsigned)shape[n]), dim3(64), 0, ctx->stream, F[n], w.Gamma, w.M,
-                         shape[n], (int)R, grads[n]);
-    }
+    float* gn = (grads && grads[n]) ? grads[n] : nullptr;
+    const bool last = n == nd - 1;
+    if (!gn && !last) continue;
+    MttkrpResult mr;
+    TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, X, nullptr, nd, shape, F, R, n, MttkrpOut::PARTIAL, nullptr,
+                            &mr));
+    if (shape[n] == 0) continue;
+    hipLaunchKernelGGL(grad_rows_partials_kernel, dim3((unsigned)shape[n]), dim3(256), 0, ctx->stream, mr.partial,
+                       mr.splits, shape[n], (int)R, w.G, nd, n, F[n], gn, last ? w.part : nullptr);
     TENRSO_HIP_TRY(hipGetLastError());
   }
-  hipLaunchKernelGGL(loss_kernel, dim3(1), dim3(256), 0, ctx->stream, w.G, nd, (int)R, w.part, npart, w.scal, w.out);
+  hipLaunchKernelGGL(loss_kernel, dim3(1), dim3(256), 0, ctx->stream, w.G, nd, (int)R, w.part, (int)shape[nd - 1],
+                     w.scal, w.out, hist);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -244,7 +288,7 @@ int cp_loss_grad(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, cons
   for (int k = 0; k < nd; ++k) total *= shape[k];
   TENRSO_TRY(fill_f64(ctx, w.scal, 64, 0.0));
   TENRSO_TRY(tensor_norm_min(ctx, X, total, w.part, w.scal));
-  TENRSO_TRY(loss_grad_device(ctx, X, nd, shape, F, R, grads, w));
+  TENRSO_TRY(loss_grad_device(ctx, X, nd, shape, F, R, grads, w, nullptr));
   double* hs = (double*)ctx->pinned;
   TENRSO_HIP_TRY(hipMemcpyAsync(hs, w.out, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -252,6 +296,8 @@ int cp_loss_grad(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, cons
   return TENRSO_OK;
 }
 
+// Gradient descent: every step is queued without a host round trip (the loss of
+// step s goes to a device history, copied to the host once at the end).
 int cp_gd(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, float* const* F, i64 R, int steps, double lr,
           double* loss_hist) {
   if (R > SOLVE_RMAX) {
@@ -265,32 +311,36 @@ int cp_gd(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, float* cons
     total *= shape[k];
     ftot += shape[k] * R;
   }
-  void* pg = nullptr;
+  void *pg = nullptr, *ph = nullptr;
   TENRSO_TRY(ctx_buffer(ctx, SLOT_STAGE_OUT, (size_t)ftot * sizeof(float), &pg));
+  if (loss_hist && steps > 0) TENRSO_TRY(ctx_buffer(ctx, SLOT_TMP2, (size_t)steps * sizeof(double), &ph));
+  double* hist_dev = (double*)ph;
   std::vector<float*> grads(nd);
+  AxpyMany am{};
   i64 off = 0;
   for (int k = 0; k < nd; ++k) {
     grads[k] = (float*)pg + off;
+    am.A[k] = F[k];
+    am.g[k] = grads[k];
+    am.off[k] = off;
     off += shape[k] * R;
   }
+  am.off[nd] = off;
   TENRSO_TRY(fill_f64(ctx, w.scal, 64, 0.0));
   TENRSO_TRY(tensor_norm_min(ctx, X, total, w.part, w.scal));
-  std::vector<double> hist_dev;
-  double* hs = (double*)ctx->pinned;
   for (int s = 0; s < steps; ++s) {
-    TENRSO_TRY(loss_grad_device(ctx, X, nd, shape, (const float* const*)F, R, grads.data(), w));
+    // loss at the point the step is taken from
+    TENRSO_TRY(loss_grad_device(ctx, X, nd, shape, (const float* const*)F, R, grads.data(), w,
+                                hist_dev ? hist_dev + s : nullptr));
     if (lr <= 0.0)
       hipLaunchKernelGGL(gamma_norms_kernel, dim3(1), dim3(256), 0, ctx->stream, w.G, nd, (int)R, w.out + 3);
-    for (int k = 0; k < nd; ++k)
-      hipLaunchKernelGGL(axpy_step_kernel, dim3(cdiv(shape[k] * R, 256)), dim3(256), 0, ctx->stream, F[k], grads[k],
-                         shape[k] * R, lr, w.out + 3, nd);
+    if (ftot > 0)
+      hipLaunchKernelGGL(axpy_step_kernel, dim3(cdiv(ftot, 256)), dim3(256), 0, ctx->stream, am, nd, lr, w.out + 3);
     TENRSO_HIP_TRY(hipGetLastError());
-    if (loss_hist) {
-      TENRSO_HIP_TRY(hipMemcpyAsync(hs, w.out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-      TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
-      loss_hist[s] = hs[0];  // loss at the point the step was taken from
-    }
   }
+  if (hist_dev)
+    TENRSO_HIP_TRY(hipMemcpyAsync(loss_hist, hist_dev, (size_t)steps * sizeof(double), hipMemcpyDeviceToHost,
+                                  ctx->stream));
   TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
   return TENRSO_OK;
 }

@@ -33,7 +33,10 @@ i64 prod(const int64_t* s, int n) {
   return p;
 }
 
-int check_mttkrp_basic(int ndim, const int64_t* shape, int64_t rank, int mode) {
+}  // namespace
+
+// the checks every MTTKRP entry point makes before touching the device
+int tenrso::mttkrp_check_args(int ndim, const int64_t* shape, int64_t rank, int mode) {
   if (ndim <= 0 || !shape) {
     set_error("tensor must have at least one mode");
     return TENRSO_ERR_INVALID_ARGUMENT;
@@ -60,6 +63,12 @@ int check_mttkrp_basic(int ndim, const int64_t* shape, int64_t rank, int mode) {
     return TENRSO_ERR_INVALID_ARGUMENT;
   }
   return TENRSO_OK;
+}
+
+namespace {
+
+int check_mttkrp_basic(int ndim, const int64_t* shape, int64_t rank, int mode) {
+  return mttkrp_check_args(ndim, shape, rank, mode);
 }
 
 int timed_begin(tenrso_ctx* ctx) {
@@ -190,18 +199,25 @@ int tenrso_mttkrp_profile(tenrso_ctx* ctx, int precision, const void* X, int ndi
     return TENRSO_OK;
   }
   TENRSO_TRY(set_device(ctx));
-  const bool bf16 = precision == TENRSO_PREC_BF16;
+  // precision 2: the three-way-split kernel of the BF16 ALS's last mode (fp32 X)
+  const bool x3 = precision == 2;
+  if (x3 && mode != ndim - 1) {
+    set_error("precision 2 profiles the last mode only");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  const bool bf16 = precision == TENRSO_PREC_BF16 || x3;
   const void* Xt = nullptr;
   if (bf16 && mode == ndim - 1) {
     const i64 total = prod(shape, ndim);
     void* pt = nullptr;
-    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * 2 + 16, &pt));
-    TENRSO_TRY(transpose_last_to_front_u16(ctx, (const uint16_t*)X, ndim, shape, (uint16_t*)pt));
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * (x3 ? 4 : 2) + 16, &pt));
+    if (x3) TENRSO_TRY(transpose_last_to_front_f32(ctx, (const float*)X, ndim, shape, (float*)pt));
+    else TENRSO_TRY(transpose_last_to_front_u16(ctx, (const uint16_t*)X, ndim, shape, (uint16_t*)pt));
     Xt = pt;
   }
   ctx->profile_reps = reps;
   int rc = mttkrp_dense(ctx, bf16 ? TENRSO_PREC_BF16 : TENRSO_PREC_FP32, X, Xt, ndim, shape, factors, rank, mode,
-                        MttkrpOut::PARTIAL, nullptr, nullptr);
+                        MttkrpOut::PARTIAL, nullptr, nullptr, nullptr, x3);
   ctx->profile_reps = 0;
   if (rc != TENRSO_OK) return rc;
   *ms = ctx->last_kernel_ms;

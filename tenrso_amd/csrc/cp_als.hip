@@ -14,7 +14,8 @@
 //   one launch   the Gram A_n^T A_n (FP64) for the modes that follow.
 // Fit: from the Grams and <X, Xhat> = sum M_{N-1} . A_{N-1} (the last mode's
 // MTTKRP, equal in exact arithmetic to the reference's separate mode-0 MTTKRP of
-// cp/helpers.rs:558-579, which fit_from_mode0 selects).
+// cp/helpers.rs:558-579, which fit_from_mode0 selects).  In BF16 mode that last
+// MTTKRP contracts the fp32 tensor exactly, so the fit is the fit to X.
 //
 // Sweep k reads factor set k % 2 and writes set (k + 1) % 2 (set 0 is the
 // caller's factors, set 1 context scratch).  So the host can queue sweep k + 1
@@ -314,13 +315,16 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
   AlsProblem pb;
   pb.nd = nd;
   pb.shape = shape;
-  // BF16: the two bf16 copies of X and its norm come from one pass over X when
+  // BF16: modes 0..N-2 contract the bf16 copy of X; the last mode, whose <M, A>
+  // gives the fit, contracts the exact fp32 tensor through its last-to-front copy
+  // (bf16x3 split on the MFMA path), so the fit is the fit to X itself.  The bf16
+  // copy, the fp32 last-to-front copy and ||X||^2 come from one pass over X when
   // the shape allows it (prepare_bf16_als); otherwise norm, copy and transposed
   // copy are three passes
   bool prepared = false;
   auto bf16_buffers = [&](void** pb16, void** pt) -> int {
     TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16, (size_t)total * 2, pb16));
-    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * 2, pt));
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * sizeof(float), pt));
     return TENRSO_OK;
   };
   pb.norm_min = [&](double* tmp, double* scal) -> int {
@@ -328,7 +332,7 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
       void *pb16, *pt;
       TENRSO_TRY(bf16_buffers(&pb16, &pt));
       int nblk = 0;
-      if (prepare_bf16_als(ctx, X, nd, shape, (uint16_t*)pb16, (uint16_t*)pt, tmp, &nblk) == TENRSO_OK) {
+      if (prepare_bf16_als(ctx, X, nd, shape, (uint16_t*)pb16, (float*)pt, tmp, &nblk) == TENRSO_OK) {
         Xk = pb16;
         Xt = pt;
         prepared = true;
@@ -342,14 +346,14 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
     void *pb16, *pt;
     TENRSO_TRY(bf16_buffers(&pb16, &pt));
     TENRSO_TRY(convert_bf16(ctx, X, nd, shape, false, (uint16_t*)pb16));
-    TENRSO_TRY(convert_bf16(ctx, X, nd, shape, true, (uint16_t*)pt));
+    TENRSO_TRY(transpose_last_to_front_f32(ctx, X, nd, shape, (float*)pt));
     Xk = pb16;
     Xt = pt;
     return TENRSO_OK;
   };
   pb.mttkrp = [&](int mode, const float* const* Fc, const AlsJob* job, MttkrpResult* mr) {
     return mttkrp_dense(ctx, bf16 ? TENRSO_PREC_BF16 : TENRSO_PREC_FP32, Xk, Xt, nd, shape, Fc, R, mode,
-                        MttkrpOut::PARTIAL, nullptr, mr, job);
+                        MttkrpOut::PARTIAL, nullptr, mr, job, /*xt_f32=*/true);
   };
   return cp_als_run(ctx, pb, F, o, res, hist);
 }

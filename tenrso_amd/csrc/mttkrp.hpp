@@ -20,14 +20,32 @@ struct MttkrpResult {
   int splits = 0;
 };
 
+// Caller-provided device workspace for the PRE Khatri-Rao rows and the split
+// partials (instead of the context's grow-only buffers).  The split count is
+// reduced until the partials fit; a shape whose partials cannot fit even
+// unsplit is TENRSO_ERR_UNSUPPORTED.  Nothing is allocated.
+struct MttkrpScratch {
+  void* base = nullptr;  // 256-byte aligned device memory
+  size_t bytes = 0;
+};
+
 // X: fp32 tensor (prec FP32) or bf16 tensor (prec BF16, uint16 storage).  For BF16
-// and mode == ndim-1 the caller may pass X_t, the tensor with the last mode moved
-// to the front (see make_bf16_copies); if X_t is null the launcher builds it.
+// and mode == ndim-1 the caller passes X_t, the tensor with the last mode moved
+// to the front: bf16 (xt_f32 false), or the exact fp32 tensor (xt_f32 true), which
+// the MFMA kernel contracts with a three-way bf16 split at FP32 accuracy.
 // job (optional): the ALS linear algebra of this mode; it runs as workgroup 0 of the
 // MTTKRP launch (or, where it cannot, as its own kernel just before it).
 int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int nd, const i64* shape,
                  const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res,
-                 const AlsJob* job = nullptr);
+                 const AlsJob* job = nullptr, bool xt_f32 = false, const MttkrpScratch* scratch = nullptr);
+
+// Size of the file-scope workspace of the no-allocation MTTKRP entry point
+// (noalloc.hip): 16 MiB per GPU.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+
+// Validation shared by the MTTKRP entry points (mode range, sizes, at least 2
+// modes), with the reference's messages.
+int mttkrp_check_args(int ndim, const int64_t* shape, int64_t rank, int mode);
 
 // The ALS job as its own one-workgroup kernel on the context stream.
 int als_job_launch(tenrso_ctx* ctx, const AlsJob& job);
@@ -38,15 +56,16 @@ int reduce_partials(tenrso_ctx* ctx, const float* partial, int splits, i64 n, fl
 // fp32 -> bf16 (RNE) copy; when `last_to_front` the output is the tensor with its
 // last mode moved to the front ([I_{N-1}, I_0, ..., I_{N-2}]).
 int convert_bf16(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, bool last_to_front, uint16_t* dst);
-// BF16 ALS input in one pass: dst = bf16(X), dstT = its last-to-front copy, and
-// (sum of squares, min) partials in tmp[0, 2 * nblk).  TENRSO_ERR_UNSUPPORTED
-// (nothing launched) unless the last mode is a multiple of 4 and X is 16-byte
-// aligned.
-int prepare_bf16_als(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, uint16_t* dst, uint16_t* dstT,
+// BF16 ALS input in one pass: dst = bf16(X), dstT = the fp32 tensor with its last
+// mode moved to the front, and (sum of squares, min) partials in tmp[0, 2 * nblk).
+// TENRSO_ERR_UNSUPPORTED (nothing launched) unless the last mode is a multiple of
+// 4 and X is 16-byte aligned.
+int prepare_bf16_als(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, uint16_t* dst, float* dstT,
                      double* tmp, int* nblk_out);
 
-// bf16 [.., I_{N-1}] -> [I_{N-1}, ..] (last mode moved to the front), plain copy.
+// [.., I_{N-1}] -> [I_{N-1}, ..] (last mode moved to the front), plain copies.
 int transpose_last_to_front_u16(tenrso_ctx* ctx, const uint16_t* src, int nd, const i64* shape, uint16_t* dst);
+int transpose_last_to_front_f32(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, float* dst);
 
 // Forward-order Khatri-Rao of `n` factors (rows = prod sizes) into dst (fp32).
 int khatri_rao_many(tenrso_ctx* ctx, int n, const float* const* F, const i64* sizes, i64 R, float* dst);

@@ -157,6 +157,45 @@ __device__ __forceinline__ void stage_window_store_bf16x2(uint16_t* __restrict__
   }
 }
 
+// Same window as an exact bf16 triple: FAST = hi + lo + lo2 (hi = bf16(FAST),
+// lo = bf16(FAST - hi), lo2 = bf16(FAST - hi - lo)); each residual is exact in
+// fp32 and the last one has at most 8 significant bits, so for every normal
+// fp32 value hi + lo + lo2 equals FAST exactly.
+template <int RT, int PITCH, int NTH, int KWMAX>
+__device__ __forceinline__ void stage_window_store_bf16x3(uint16_t* __restrict__ shi, uint16_t* __restrict__ slo,
+                                                          uint16_t* __restrict__ slo2,
+                                                          const FastWindow<RT, NTH, KWMAX>& fw) {
+  static_assert(PITCH % 2 == 0, "pairs of window rows are stored as 4-byte words");
+#pragma unroll
+  for (int e = 0; e < FastWindow<RT, NTH, KWMAX>::PERU; ++e) {
+    const int u = threadIdx.x + e * NTH;
+    const int r4 = 4 * (u % (RT / 4)), kk = 2 * (u / (RT / 4));
+    if (!FastWindow<RT, NTH, KWMAX>::live(u)) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const floatx2 f = {fw.v[e][0][q], fw.v[e][1][q]};
+      const uint32_t hw = __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2));
+      const floatx2 r = {f[0] - __uint_as_float(hw << 16), f[1] - __uint_as_float(hw & 0xffff0000u)};
+      const uint32_t lw = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
+      const floatx2 r2 = {r[0] - __uint_as_float(lw << 16), r[1] - __uint_as_float(lw & 0xffff0000u)};
+      const uint32_t l2w = __builtin_bit_cast(uint32_t, __builtin_convertvector(r2, bf16x2));
+      const int o = (r4 + q) * PITCH + kk;
+      *reinterpret_cast<uint32_t*>(&shi[o]) = hw;
+      *reinterpret_cast<uint32_t*>(&slo[o]) = lw;
+      *reinterpret_cast<uint32_t*>(&slo2[o]) = l2w;
+    }
+  }
+}
+
+// Exact three-way bf16 split of 8 fp32 values (see stage_window_store_bf16x3).
+__device__ __forceinline__ void split_bf16x3(const floatx8& x, bf16x8& h, bf16x8& l, bf16x8& l2) {
+  h = __builtin_convertvector(x, bf16x8);
+  const floatx8 r = x - __builtin_convertvector(h, floatx8);
+  l = __builtin_convertvector(r, bf16x8);
+  const floatx8 r2 = r - __builtin_convertvector(l, floatx8);
+  l2 = __builtin_convertvector(r2, bf16x8);
+}
+
 // A value the compiler cannot prove uniform but that is: move it to scalar registers.
 __device__ __forceinline__ i64 uniform_i64(i64 v) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
@@ -468,16 +507,25 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
 // ---------------------------------------------------------------------------
 // BF16 MFMA kernel (v_mfma_f32_16x16x32_bf16), case A only (q-contiguous X; the
 // last mode runs on the last-to-front copy).  Same work split and the same
-// T-then-PRE contraction as the FP32 kernel.  X enters as bf16 (one 16-byte load
-// of 8 consecutive j per lane and k-step); the FAST window enters as an exact
-// bf16 hi + lo pair (two MFMAs per step), so the input tensor is the only
-// operand rounded to bf16.  Accumulation is FP32.
+// T-then-PRE contraction as the FP32 kernel.  Accumulation is FP32.
+//   XIN = uint16_t: X enters as bf16 (one 16-byte load of 8 consecutive j per
+//     lane and k-step); the FAST window enters as an exact bf16 hi + lo pair (two
+//     MFMAs per step), so the input tensor is the only operand rounded to bf16.
+//   XIN = float ("bf16x3"): X enters as fp32 (two 16-byte loads per lane and
+//     k-step) and is split in registers into an exact bf16 triple, as is the FAST
+//     window; six MFMAs per step form every product term down to 2^-16 of the
+//     leading one (x_h f_h; x_h f_l, x_l f_h; x_h f_l2, x_l f_l, x_l2 f_h), so the
+//     result has the accuracy of an FP32 MTTKRP of the fp32 tensor.  The BF16 ALS
+//     runs its last mode, whose <M, A> gives the fit, this way.
 // ---------------------------------------------------------------------------
-template <int NT, int WAVES, int MT, int KS, int KWMAX, bool WV>
-__global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* __restrict__ X, KernArgs a,
+template <typename XIN, int NT, int WAVES, int MT, int KS, int KWMAX, bool WV>
+__global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __restrict__ X, KernArgs a,
                                                                float* __restrict__ partial, AlsJob job, int has_job) {
+  constexpr bool X3 = std::is_same<XIN, float>::value;
+  static_assert(!X3 || KS == 1, "fp32 input uses one 32-wide k-step per block (register budget)");
+  constexpr int NWIN = X3 ? 3 : 2;  // bf16 parts of the FAST window
   constexpr int KB = 32 * KS, PITCH = KWMAX + 8, RT = 16 * NT, NTH = WAVES * 64, ROWS = 16 * MT;
-  constexpr int LDS_WIN_U16 = 2 * RT * PITCH, LDS_RED_F = MT * NT * WAVES * 64 * 4;
+  constexpr int LDS_WIN_U16 = NWIN * RT * PITCH, LDS_RED_F = MT * NT * WAVES * 64 * 4;
   constexpr int LDS_B = (LDS_WIN_U16 * 2 > LDS_RED_F * 4) ? LDS_WIN_U16 * 2 : LDS_RED_F * 4;
   constexpr int LDS_BYTES = LDS_B > JOB_LDS_BYTES ? LDS_B : JOB_LDS_BYTES;
   __shared__ __attribute__((aligned(16))) float smem[LDS_BYTES / 4];
@@ -503,6 +551,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
   const int xt = tile % a.rtiles;
   uint16_t* sHi = reinterpret_cast<uint16_t*>(smem);
   uint16_t* sLo = sHi + RT * PITCH;
+  uint16_t* sLo2 = sLo + RT * PITCH;  // X3 only
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar schedule and addresses
   const int g = lane >> 4, c = lane & 15;
@@ -539,8 +588,9 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
       Tacc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
 
+  using XV = std::conditional_t<X3, floatx8, bf16x8>;
   struct Buf {
-    bf16x8 x[MT][KS];
+    XV x[MT][KS];
     float pre[NT];
     int kb;
   };
@@ -567,12 +617,20 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) b.pre[nt] = prow[poff[nt]];
     const bool live = t < T;  // past the end: one cache line at X (see the FP32 kernel)
-    const uint16_t* ubase = X + (live ? cur.offset(a) : 0);
+    const XIN* ubase = X + (live ? cur.offset(a) : 0);
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
-      const uint16_t* base = ubase + (live ? loff[mt] : 0);
+      const XIN* base = ubase + (live ? loff[mt] : 0);
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) b.x[mt][ks] = *reinterpret_cast<const bf16x8*>(base + 32 * ks);
+      for (int ks = 0; ks < KS; ++ks) {
+        if constexpr (X3) {
+          const floatx4 v0 = *reinterpret_cast<const floatx4*>(base + 32 * ks);
+          const floatx4 v1 = *reinterpret_cast<const floatx4*>(base + 32 * ks + 4);
+          b.x[mt][ks] = floatx8{v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
+        } else {
+          b.x[mt][ks] = *reinterpret_cast<const bf16x8*>(base + 32 * ks);
+        }
+      }
     }
     if (t + 1 < T) cur.step(a);
   };
@@ -580,18 +638,43 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
   auto compute = [&](int t, const Buf& b) {
     if (t >= T) return;
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks)
+    for (int ks = 0; ks < KS; ++ks) {
+      if constexpr (X3) {
+        bf16x8 xh[MT], xl[MT], xl2[MT];
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        const int off = (16 * nt + c) * PITCH + b.kb * KB + 32 * ks + 8 * g;
-        const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&sHi[off]);
-        const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&sLo[off]);
+        for (int mt = 0; mt < MT; ++mt) split_bf16x3(b.x[mt][ks], xh[mt], xl[mt], xl2[mt]);
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b.x[mt][ks], bh, Tacc[mt][nt], 0, 0, 0);
-          Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b.x[mt][ks], bl, Tacc[mt][nt], 0, 0, 0);
+        for (int nt = 0; nt < NT; ++nt) {
+          const int off = (16 * nt + c) * PITCH + b.kb * KB + 32 * ks + 8 * g;
+          const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&sHi[off]);
+          const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&sLo[off]);
+          const bf16x8 bl2 = *reinterpret_cast<const bf16x8*>(&sLo2[off]);
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            // smallest terms first
+            floatx4 d = Tacc[mt][nt];
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl2[mt], bh, d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[mt], bl, d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[mt], bl2, d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[mt], bh, d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[mt], bl, d, 0, 0, 0);
+            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[mt], bh, d, 0, 0, 0);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const int off = (16 * nt + c) * PITCH + b.kb * KB + 32 * ks + 8 * g;
+          const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&sHi[off]);
+          const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&sLo[off]);
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b.x[mt][ks], bh, Tacc[mt][nt], 0, 0, 0);
+            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b.x[mt][ks], bl, Tacc[mt][nt], 0, 0, 0);
+          }
         }
       }
+    }
     // a slow index ends, or this wave's range does (it may end inside one)
     if (b.kb == nkb - 1 || t == T - 1) fold_slow<MT, NT>(acc, Tacc, b.pre);
   };
@@ -600,7 +683,8 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const uint16_t* _
   load(0, b0);
   load(1, b1);
   load(2, b2);
-  stage_window_store_bf16x2<RT, PITCH, NTH, KWMAX>(sHi, sLo, fw);
+  if constexpr (X3) stage_window_store_bf16x3<RT, PITCH, NTH, KWMAX>(sHi, sLo, sLo2, fw);
+  else stage_window_store_bf16x2<RT, PITCH, NTH, KWMAX>(sHi, sLo, fw);
   __syncthreads();  // LDS window staged
   for (int t = 0; t < T; t += 4) {
     load(t + 3, b3);
@@ -665,14 +749,43 @@ __global__ __launch_bounds__(256) void mttkrp_generic(const void* __restrict__ X
 // ---------------------------------------------------------------------------
 // Reduction of split partials (fixed order, FP64) and conversions.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ part, int splits, i64 n,
-                                                              float* __restrict__ o32, double* __restrict__ o64) {
-  const i64 idx = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= n) return;
+// A workgroup owns 64 consecutive outputs and splits the split range into
+// RED_GROUPS contiguous pieces, one per group of 64 threads (each lane one
+// output, so every load of a group is one contiguous 256-byte row segment).
+// Each group sums its piece in FP64 with four independent chains; the group
+// sums are then added in group order.  The order is fixed, so the result is
+// run-to-run identical.  (One thread per output looping over all splits left
+// 12 workgroups for a 96^4 rank-32 result, a latency chain of ~170 loads.)
+constexpr int RED_GROUPS = 16;
+__global__ __launch_bounds__(64 * RED_GROUPS) void reduce_partials_kernel(const float* __restrict__ part, int splits,
+                                                                          i64 n, float* __restrict__ o32,
+                                                                          double* __restrict__ o64) {
+  __shared__ double red[RED_GROUPS][64];
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const i64 idx = (i64)blockIdx.x * 64 + lane;
   double s = 0.0;
-  for (int sp = 0; sp < splits; ++sp) s += (double)part[(i64)sp * n + idx];
-  if (o32) o32[idx] = (float)s;
-  if (o64) o64[idx] = s;
+  if (idx < n) {
+    const int s0 = (int)((i64)grp * splits / RED_GROUPS), s1 = (int)((i64)(grp + 1) * splits / RED_GROUPS);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int sp = s0;
+    for (; sp + 4 <= s1; sp += 4) {
+      a0 += (double)part[(i64)(sp + 0) * n + idx];
+      a1 += (double)part[(i64)(sp + 1) * n + idx];
+      a2 += (double)part[(i64)(sp + 2) * n + idx];
+      a3 += (double)part[(i64)(sp + 3) * n + idx];
+    }
+    for (; sp < s1; ++sp) a0 += (double)part[(i64)sp * n + idx];
+    s = (a0 + a1) + (a2 + a3);
+  }
+  red[grp][lane] = s;
+  __syncthreads();
+  if (grp == 0 && idx < n) {
+    double t = red[0][lane];
+#pragma unroll
+    for (int g = 1; g < RED_GROUPS; ++g) t += red[g][lane];
+    if (o32) o32[idx] = (float)t;
+    if (o64) o64[idx] = t;
+  }
 }
 
 // vec: src 16-byte and dst 8-byte aligned (checked on the host), else scalar
@@ -692,14 +805,15 @@ __global__ __launch_bounds__(256) void f32_to_bf16_kernel(const float* __restric
   }
 }
 
-// [rows, K] -> [K, rows] transpose through a 32x32 LDS tile, writing bf16
-// (from fp32 with RNE rounding, or a plain copy of bf16 input).
-__device__ __forceinline__ uint16_t to_bf16(float v) { return f32_to_bf16_rne(v); }
-__device__ __forceinline__ uint16_t to_bf16(uint16_t v) { return v; }
+// [rows, K] -> [K, rows] transpose through a 32x32 LDS tile: fp32 -> bf16 (RNE),
+// bf16 -> bf16 or fp32 -> fp32.
+__device__ __forceinline__ uint16_t to_out(float v, uint16_t*) { return f32_to_bf16_rne(v); }
+__device__ __forceinline__ uint16_t to_out(uint16_t v, uint16_t*) { return v; }
+__device__ __forceinline__ float to_out(float v, float*) { return v; }
 
-template <typename TIN>
-__global__ __launch_bounds__(256) void transpose_bf16_kernel(const TIN* __restrict__ src, i64 rows, i64 K,
-                                                             uint16_t* __restrict__ dst) {
+template <typename TIN, typename TOUT>
+__global__ __launch_bounds__(256) void transpose_kernel(const TIN* __restrict__ src, i64 rows, i64 K,
+                                                        TOUT* __restrict__ dst) {
   __shared__ TIN tile[32][33];
   const i64 r0 = (i64)blockIdx.y * 32, k0 = (i64)blockIdx.x * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -710,7 +824,7 @@ __global__ __launch_bounds__(256) void transpose_bf16_kernel(const TIN* __restri
   __syncthreads();
   for (int yy = ty; yy < 32; yy += 8) {
     const i64 k = k0 + yy, r = r0 + tx;
-    if (k < K && r < rows) dst[k * rows + r] = to_bf16(tile[tx][yy]);
+    if (k < K && r < rows) dst[k * rows + r] = to_out(tile[tx][yy], dst);
   }
 }
 
@@ -744,8 +858,8 @@ static inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); 
 
 int reduce_partials(tenrso_ctx* ctx, const float* partial, int splits, i64 n, float* out32, double* out64) {
   if (n == 0) return TENRSO_OK;
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(cdiv(n, 256)), dim3(256), 0, ctx->stream, partial, splits, n,
-                     out32, out64);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(cdiv(n, 64)), dim3(64 * RED_GROUPS), 0, ctx->stream, partial,
+                     splits, n, out32, out64);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -775,24 +889,36 @@ int convert_bf16(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, bo
                        dst, vec);
   } else {
     const i64 K = shape[nd - 1], rows = total / K;
-    hipLaunchKernelGGL(transpose_bf16_kernel<float>, dim3(cdiv(K, 32), cdiv(rows, 32)), dim3(256), 0, ctx->stream,
-                       src, rows, K, dst);
+    hipLaunchKernelGGL((transpose_kernel<float, uint16_t>), dim3(cdiv(K, 32), cdiv(rows, 32)), dim3(256), 0,
+                       ctx->stream, src, rows, K, dst);
   }
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
 
+int transpose_last_to_front_f32(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, float* dst) {
+  i64 total = 1;
+  for (int k = 0; k < nd; ++k) total *= shape[k];
+  if (total == 0) return TENRSO_OK;
+  const i64 K = shape[nd - 1], rows = total / K;
+  hipLaunchKernelGGL((transpose_kernel<float, float>), dim3(cdiv(K, 32), cdiv(rows, 32)), dim3(256), 0, ctx->stream,
+                     src, rows, K, dst);
+  TENRSO_HIP_TRY(hipGetLastError());
+  return TENRSO_OK;
+}
+
 // One pass over an fp32 tensor viewed as [rows, K] (K = last mode) for the BF16
-// ALS: the bf16 copy (same layout), the last-to-front bf16 copy ([K, rows]) and
+// ALS: the bf16 copy (same layout), the last-to-front fp32 copy ([K, rows]; the
+// last mode's MTTKRP contracts the exact tensor, see mttkrp_bf16_mfma) and
 // per-workgroup (sum of squares, min) partials, instead of three passes.  64 x 64
 // tiles, grid-stride over tiles; each thread loads 4 x 16 bytes (K % 4 == 0, X
 // 16-byte aligned), writes its bf16 quads straight out and through an LDS tile
-// the transposed quads.  The bf16 values are f32_to_bf16_rne, as in the
-// separate conversion kernels.
+// the transposed fp32 quads.  The bf16 values are f32_to_bf16_rne, as in the
+// separate conversion kernel.
 __global__ __launch_bounds__(256) void prepare_bf16_kernel(const float* __restrict__ X, i64 rows, i64 K,
-                                                           uint16_t* __restrict__ dst, uint16_t* __restrict__ dstT,
+                                                           uint16_t* __restrict__ dst, float* __restrict__ dstT,
                                                            double* __restrict__ part) {
-  __shared__ uint16_t tile[64][64 + 4];
+  __shared__ float tile[64][64 + 1];
   __shared__ double red[256];
   const int tid = threadIdx.x;
   const i64 tr_n = (rows + 63) / 64, tk_n = (K + 63) / 64;
@@ -806,24 +932,23 @@ __global__ __launch_bounds__(256) void prepare_bf16_kernel(const float* __restri
     for (int j = 0; j < 4; ++j) {
       const int rr = (tid >> 4) + 16 * j;
       const i64 r = r0 + rr, k = k0 + c4;
-      ushort4 o = {0, 0, 0, 0};
+      floatx4 v = {0.f, 0.f, 0.f, 0.f};
       if (r < rows && k < K) {
-        const floatx4 v = *reinterpret_cast<const floatx4*>(X + r * K + k);
+        v = *reinterpret_cast<const floatx4*>(X + r * K + k);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           ss = fma((double)v[q], (double)v[q], ss);
           mn = fminf(mn, v[q]);
         }
+        ushort4 o;
         o.x = f32_to_bf16_rne(v[0]);
         o.y = f32_to_bf16_rne(v[1]);
         o.z = f32_to_bf16_rne(v[2]);
         o.w = f32_to_bf16_rne(v[3]);
         *reinterpret_cast<ushort4*>(dst + r * K + k) = o;
       }
-      tile[rr][c4] = o.x;
-      tile[rr][c4 + 1] = o.y;
-      tile[rr][c4 + 2] = o.z;
-      tile[rr][c4 + 3] = o.w;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) tile[rr][c4 + q] = v[q];
     }
     __syncthreads();
     // transposed quads: thread -> (column kk = tid / 16 + 16 j, rows 4 (tid % 16) ..)
@@ -832,15 +957,15 @@ __global__ __launch_bounds__(256) void prepare_bf16_kernel(const float* __restri
       const int kk = (tid >> 4) + 16 * j, r4 = 4 * (tid & 15);
       const i64 k = k0 + kk, r = r0 + r4;
       if (k < K && r < rows) {
-        const ushort4 o = {tile[r4][kk], tile[r4 + 1][kk], tile[r4 + 2][kk], tile[r4 + 3][kk]};
-        uint16_t* d = dstT + k * rows + r;
+        const floatx4 o = {tile[r4][kk], tile[r4 + 1][kk], tile[r4 + 2][kk], tile[r4 + 3][kk]};
+        float* d = dstT + k * rows + r;
         if (r + 3 < rows && rows % 4 == 0) {
-          *reinterpret_cast<ushort4*>(d) = o;
+          *reinterpret_cast<floatx4*>(d) = o;
         } else {
-          d[0] = o.x;
-          if (r + 1 < rows) d[1] = o.y;
-          if (r + 2 < rows) d[2] = o.z;
-          if (r + 3 < rows) d[3] = o.w;
+          d[0] = o[0];
+          if (r + 1 < rows) d[1] = o[1];
+          if (r + 2 < rows) d[2] = o[2];
+          if (r + 3 < rows) d[3] = o[3];
         }
       }
     }
@@ -865,7 +990,7 @@ __global__ __launch_bounds__(256) void prepare_bf16_kernel(const float* __restri
   }
 }
 
-int prepare_bf16_als(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, uint16_t* dst, uint16_t* dstT,
+int prepare_bf16_als(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, uint16_t* dst, float* dstT,
                      double* tmp, int* nblk_out) {
   i64 total = 1;
   for (int k = 0; k < nd; ++k) total *= shape[k];
@@ -885,8 +1010,8 @@ int transpose_last_to_front_u16(tenrso_ctx* ctx, const uint16_t* src, int nd, co
   for (int k = 0; k < nd; ++k) total *= shape[k];
   if (total == 0) return TENRSO_OK;
   const i64 K = shape[nd - 1], rows = total / K;
-  hipLaunchKernelGGL(transpose_bf16_kernel<uint16_t>, dim3(cdiv(K, 32), cdiv(rows, 32)), dim3(256), 0, ctx->stream,
-                     src, rows, K, dst);
+  hipLaunchKernelGGL((transpose_kernel<uint16_t, uint16_t>), dim3(cdiv(K, 32), cdiv(rows, 32)), dim3(256), 0,
+                     ctx->stream, src, rows, K, dst);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -928,7 +1053,7 @@ Plan make_plan(int nd, const i64* shape, int mode) {
 
 int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int nd, const i64* shape,
                  const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res,
-                 const AlsJob* job) {
+                 const AlsJob* job, bool xt_f32, const MttkrpScratch* scratch) {
   Plan pl = make_plan(nd, shape, mode);
   const bool bf16 = prec == TENRSO_PREC_BF16;
   const float* Fv[MAXD];
@@ -938,8 +1063,11 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
   i64 shape_use[MAXD];
   for (int k = 0; k < nd; ++k) shape_use[k] = shape[k];
 
-  // BF16 last mode: run mode 0 of the last-to-front copy (same j order).
+  // BF16 last mode: run mode 0 of the last-to-front copy (same j order); an
+  // fp32 copy is contracted exactly (bf16x3 split)
+  bool x3 = false;
   if (bf16 && mode == nd - 1 && nd >= 2) {
+    x3 = xt_f32;
     if (!X_t) {
       set_error("internal: bf16 last-mode MTTKRP needs the last-to-front copy");
       return TENRSO_ERR_INVALID_ARGUMENT;
@@ -963,6 +1091,28 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
 
   // PRE: product of the other modes except the fastest (forward order).
   const float* pre = nullptr;
+  size_t scratch_used = 0;  // bytes of the caller's workspace taken by PRE
+  // the most split partials that fit in the caller's workspace (or no bound)
+  auto max_splits = [&]() -> i64 {
+    if (!scratch) return INT64_MAX;
+    const size_t per = (size_t)std::max<i64>(1, nout) * sizeof(float);
+    return scratch->bytes > scratch_used ? (i64)((scratch->bytes - scratch_used) / per) : 0;
+  };
+  auto partial_buffer = [&](int nsplits, float** p) -> int {
+    if (scratch) {
+      *p = reinterpret_cast<float*>(static_cast<char*>(scratch->base) + scratch_used);
+      return TENRSO_OK;
+    }
+    void* pbuf = nullptr;
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_PARTIAL, (size_t)nsplits * nout * sizeof(float), &pbuf));
+    *p = (float*)pbuf;
+    return TENRSO_OK;
+  };
+  auto too_small = [&](i64 need_splits) {
+    set_error("workspace of %zu bytes is too small for this MTTKRP (needs %zu bytes)", scratch->bytes,
+              scratch_used + (size_t)need_splits * (size_t)std::max<i64>(1, nout) * sizeof(float));
+    return TENRSO_ERR_UNSUPPORTED;
+  };
   if (pl.n_oth >= 3) {
     const float* pf[MAXD];
     i64 ps[MAXD];
@@ -973,7 +1123,17 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
       prows *= ps[m];
     }
     void* buf = nullptr;
-    TENRSO_TRY(ctx_buffer(ctx, SLOT_TMP, (size_t)(prows * R) * sizeof(float), &buf));
+    const size_t pre_bytes = (size_t)(prows * R) * sizeof(float);
+    if (scratch) {
+      if (pre_bytes > scratch->bytes) {
+        set_error("workspace of %zu bytes cannot hold the %zu bytes of Khatri-Rao rows", scratch->bytes, pre_bytes);
+        return TENRSO_ERR_UNSUPPORTED;
+      }
+      buf = scratch->base;
+      scratch_used = (pre_bytes + 255) / 256 * 256;
+    } else {
+      TENRSO_TRY(ctx_buffer(ctx, SLOT_TMP, pre_bytes, &buf));
+    }
     TENRSO_TRY(khatri_rao_many(ctx, pl.n_oth - 1, pf, ps, R, (float*)buf));
     pre = (const float*)buf;
   } else if (pl.n_oth == 2) {
@@ -990,7 +1150,7 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
   a.fast = fastf;
 
   const bool caseB = pl.Q == 1;
-  const int KB = bf16 ? ((pl.fast_size % 64 == 0) ? 64 : 32) : 32;
+  const int KB = (bf16 && !x3) ? ((pl.fast_size % 64 == 0) ? 64 : 32) : 32;
   constexpr int KWMAX = 128;  // LDS window bound (fp32: 64 x 132 floats)
   // LDS window of the fastest mode: the largest multiple of KB that divides it
   int kw = 0;
@@ -1042,14 +1202,19 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
           if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 1, true, KWMAX, WV>);
           return f(mttkrp_f32_mfma<4, WAVES, 1, true, KWMAX, WV>);
         }
-        if (KB == 64) {
-          if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES_BF16, 2, 2, KWMAX, WV>);
-          if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES_BF16, 2, 2, KWMAX, WV>);
-          return f(mttkrp_bf16_mfma<4, WAVES_BF16, 2, 2, KWMAX, WV>);
+        if (x3) {
+          if (NT == 1) return f(mttkrp_bf16_mfma<float, 1, WAVES_BF16, 2, 1, KWMAX, WV>);
+          if (NT == 2) return f(mttkrp_bf16_mfma<float, 2, WAVES_BF16, 2, 1, KWMAX, WV>);
+          return f(mttkrp_bf16_mfma<float, 4, WAVES_BF16, 2, 1, KWMAX, WV>);
         }
-        if (NT == 1) return f(mttkrp_bf16_mfma<1, WAVES_BF16, 2, 1, KWMAX, WV>);
-        if (NT == 2) return f(mttkrp_bf16_mfma<2, WAVES_BF16, 2, 1, KWMAX, WV>);
-        return f(mttkrp_bf16_mfma<4, WAVES_BF16, 2, 1, KWMAX, WV>);
+        if (KB == 64) {
+          if (NT == 1) return f(mttkrp_bf16_mfma<uint16_t, 1, WAVES_BF16, 2, 2, KWMAX, WV>);
+          if (NT == 2) return f(mttkrp_bf16_mfma<uint16_t, 2, WAVES_BF16, 2, 2, KWMAX, WV>);
+          return f(mttkrp_bf16_mfma<uint16_t, 4, WAVES_BF16, 2, 2, KWMAX, WV>);
+        }
+        if (NT == 1) return f(mttkrp_bf16_mfma<uint16_t, 1, WAVES_BF16, 2, 1, KWMAX, WV>);
+        if (NT == 2) return f(mttkrp_bf16_mfma<uint16_t, 2, WAVES_BF16, 2, 1, KWMAX, WV>);
+        return f(mttkrp_bf16_mfma<uint16_t, 4, WAVES_BF16, 2, 1, KWMAX, WV>);
       };
       return a.win_vec ? pick(std::true_type{}) : pick(std::false_type{});
     };
@@ -1087,13 +1252,14 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     i64 ssp = std::max<i64>(1, target / std::max<i64>(1, tiles) / a.nwin);
     ssp = std::min<i64>(ssp, a.n_slow >= waves ? a.n_slow / waves : 1);
     ssp = std::min<i64>(ssp, 65535 / a.nwin);
+    // a caller's workspace bounds the partials: fewer splits, or refuse
+    if (max_splits() < a.nwin) return too_small(a.nwin);
+    ssp = std::min<i64>(ssp, max_splits() / a.nwin);
     a.slow_splits = (int)std::max<i64>(1, ssp);
     a.slow_q = a.n_slow / a.slow_splits;
     a.slow_rem = a.n_slow % a.slow_splits;
     splits = a.nwin * a.slow_splits;
-    void* pbuf = nullptr;
-    TENRSO_TRY(ctx_buffer(ctx, SLOT_PARTIAL, (size_t)splits * nout * sizeof(float), &pbuf));
-    partial = (float*)pbuf;
+    TENRSO_TRY(partial_buffer(splits, &partial));
     if (job && !fuse_job) TENRSO_TRY(als_job_launch(ctx, *job));
     const dim3 block(waves * 64);
     // BF16 with the job fused: the MTTKRP workgroups wait for the job's Gram
@@ -1155,14 +1321,14 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
       ga.oth_size[m] = pl.oth_size[m];
       ga.oth_fac[m] = Fv[pl.oth[m]];
     }
-    ga.bf16 = bf16 ? 1 : 0;
+    ga.bf16 = (bf16 && !x3) ? 1 : 0;
     const i64 tiles = cdiv(nout, 256);
     splits = (int)std::max<i64>(1, std::min<i64>(pl.J, (2LL * ctx->num_cus + tiles - 1) / std::max<i64>(1, tiles)));
     splits = std::min(splits, 1024);
+    if (max_splits() < 1) return too_small(1);
+    splits = (int)std::min<i64>(splits, max_splits());
     ga.splits = splits;
-    void* pbuf = nullptr;
-    TENRSO_TRY(ctx_buffer(ctx, SLOT_PARTIAL, (size_t)splits * nout * sizeof(float), &pbuf));
-    partial = (float*)pbuf;
+    TENRSO_TRY(partial_buffer(splits, &partial));
     if (nout > 0) {
       hipLaunchKernelGGL(mttkrp_generic, dim3((unsigned)tiles, splits), dim3(256), 0, ctx->stream, Xuse, ga,
                          partial);

@@ -106,6 +106,20 @@ __global__ __launch_bounds__(256) void bounds_kernel(KeyArgs a, int* __restrict_
   }
 }
 
+// Device temporaries of one build step, freed on every path out of it.
+struct DevTemps {
+  std::vector<void*> ptrs;
+  int alloc(void** p, size_t bytes) {
+    *p = nullptr;
+    TENRSO_HIP_TRY(hipMalloc(p, bytes));
+    ptrs.push_back(*p);
+    return TENRSO_OK;
+  }
+  ~DevTemps() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
+
 int sp_build_mode(tenrso_ctx* ctx, tenrso_sptensor* t, int mode) {
   SpMode& md = t->modes[mode];
   const i64 nnz = t->nnz;
@@ -115,12 +129,13 @@ int sp_build_mode(tenrso_ctx* ctx, tenrso_sptensor* t, int mode) {
   TENRSO_HIP_TRY(hipMalloc(&md.oth, (size_t)std::max<i64>(1, nnz * (t->nd - 1)) * sizeof(int32_t)));
   TENRSO_HIP_TRY(hipMalloc(&md.vals, (size_t)std::max<i64>(1, nnz) * sizeof(float)));
   if (nnz > 0) {
+    DevTemps tmps;  // freed when this block ends, also on an early error return
     unsigned long long *kin, *kout;
     int32_t *pin, *pout;
-    TENRSO_HIP_TRY(hipMalloc(&kin, (size_t)nnz * 8));
-    TENRSO_HIP_TRY(hipMalloc(&kout, (size_t)nnz * 8));
-    TENRSO_HIP_TRY(hipMalloc(&pin, (size_t)nnz * 4));
-    TENRSO_HIP_TRY(hipMalloc(&pout, (size_t)nnz * 4));
+    TENRSO_TRY(tmps.alloc((void**)&kin, (size_t)nnz * 8));
+    TENRSO_TRY(tmps.alloc((void**)&kout, (size_t)nnz * 8));
+    TENRSO_TRY(tmps.alloc((void**)&pin, (size_t)nnz * 4));
+    TENRSO_TRY(tmps.alloc((void**)&pout, (size_t)nnz * 4));
     KeyArgs ka{};
     ka.nd = t->nd;
     ka.mode = mode;
@@ -139,22 +154,46 @@ int sp_build_mode(tenrso_ctx* ctx, tenrso_sptensor* t, int mode) {
     TENRSO_HIP_TRY(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, pin, pout, (size_t)nnz, 0, bits,
                                              ctx->stream));
     void* tmp = nullptr;
-    TENRSO_HIP_TRY(hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16)));
+    TENRSO_TRY(tmps.alloc(&tmp, std::max<size_t>(tmp_bytes, 16)));
     TENRSO_HIP_TRY(rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, pin, pout, (size_t)nnz, 0, bits,
                                              ctx->stream));
     GatherArgs ga{t->nd, mode, nnz, t->idx, t->vals, pout, md.row, md.oth, md.vals};
     hipLaunchKernelGGL(gather_sorted_kernel, dim3(cdiv(nnz, 256)), dim3(256), 0, ctx->stream, ga);
     TENRSO_HIP_TRY(hipGetLastError());
+    // the temporaries are freed below: the kernels reading them must be done
     TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    TENRSO_HIP_TRY(hipFree(tmp));
-    TENRSO_HIP_TRY(hipFree(kin));
-    TENRSO_HIP_TRY(hipFree(kout));
-    TENRSO_HIP_TRY(hipFree(pin));
-    TENRSO_HIP_TRY(hipFree(pout));
   }
   hipLaunchKernelGGL(rowptr_kernel, dim3(cdiv(md.rows + 1, 256)), dim3(256), 0, ctx->stream, md.row, nnz, md.rows,
                      md.rowptr);
   TENRSO_HIP_TRY(hipGetLastError());
+  return TENRSO_OK;
+}
+
+// Index bounds of CooTensor::new (coo.rs:127-139) on the device.  Fails closed:
+// any HIP error is reported as a device error, never as "in bounds".  The flag is
+// cleared by a fill kernel, not a memset.
+__global__ void clear_flag_kernel(int* __restrict__ f) { f[0] = 0; }
+
+static int check_index_bounds(tenrso_ctx* ctx, int nd, const i64* shape, i64 nnz, const int32_t* idx) {
+  DevTemps tmps;
+  int* bad = nullptr;
+  TENRSO_TRY(tmps.alloc((void**)&bad, sizeof(int)));
+  hipLaunchKernelGGL(clear_flag_kernel, dim3(1), dim3(1), 0, ctx->stream, bad);
+  TENRSO_HIP_TRY(hipGetLastError());
+  KeyArgs ka{};
+  ka.nd = nd;
+  ka.nnz = nnz;
+  for (int k = 0; k < nd; ++k) ka.shape[k] = shape[k];
+  ka.idx = idx;
+  hipLaunchKernelGGL(bounds_kernel, dim3(cdiv(nnz, 256)), dim3(256), 0, ctx->stream, ka, bad);
+  TENRSO_HIP_TRY(hipGetLastError());
+  int hbad = 1;
+  TENRSO_HIP_TRY(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (hbad) {
+    set_error("Index out of bounds");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
   return TENRSO_OK;
 }
 
@@ -210,23 +249,8 @@ int sp_create(tenrso_ctx* ctx, int nd, const i64* shape, i64 nnz, const int32_t*
       return fail(TENRSO_ERR_DEVICE);
     }
     // index bounds (CooTensor::new, coo.rs:127-139)
-    int* bad = nullptr;
-    if (hipMalloc(&bad, sizeof(int)) != hipSuccess) return fail(TENRSO_ERR_DEVICE);
-    (void)hipMemsetAsync(bad, 0, sizeof(int), ctx->stream);
-    KeyArgs ka{};
-    ka.nd = nd;
-    ka.nnz = nnz;
-    for (int k = 0; k < nd; ++k) ka.shape[k] = shape[k];
-    ka.idx = t->idx;
-    hipLaunchKernelGGL(bounds_kernel, dim3(cdiv(nnz, 256)), dim3(256), 0, ctx->stream, ka, bad);
-    int hbad = 0;
-    (void)hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(bad);
-    if (hbad) {
-      set_error("Index out of bounds");
-      return fail(TENRSO_ERR_INVALID_ARGUMENT);
-    }
+    rc = check_index_bounds(ctx, nd, shape, nnz, t->idx);
+    if (rc != TENRSO_OK) return fail(rc);
   }
   for (int m = 0; m < nd; ++m) {
     rc = sp_build_mode(ctx, t, m);

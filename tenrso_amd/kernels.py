@@ -31,6 +31,28 @@ def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
 
 
+def _is_cuda(x) -> bool:
+    return _is_torch(x) and bool(x.is_cuda)
+
+
+def _to_numpy(x):
+    """A torch tensor (any device) as a host numpy array (other inputs unchanged)."""
+    return x.detach().cpu().numpy() if _is_torch(x) else x
+
+
+def _dev_f32(t, device):
+    """`t` (numpy array or torch tensor, any float dtype, any device) as a
+    contiguous float32 CUDA tensor on `device`: what the device entry points read
+    through raw pointers.  Factors of another dtype or on another device (the
+    reference's usual T = f64, CPU tensors) are converted, never reinterpreted."""
+    import torch
+    if _is_torch(t):
+        if t.is_complex():
+            raise TypeError("complex factors are not supported")
+        return t.detach().to(device=device, dtype=torch.float32).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(t, dtype=np.float32), device=device)
+
+
 def _i64s(vals):
     return (C.c_int64 * len(vals))(*[int(v) for v in vals])
 
@@ -59,12 +81,16 @@ def mttkrp(tensor, factors, mode: int):
     """
     shape = tuple(int(s) for s in tensor.shape)
     R = _check_mttkrp(shape, factors, mode)
+    if _is_torch(tensor) and not tensor.is_cuda:
+        # a torch CPU tensor takes the host path and gets a torch CPU result
+        import torch
+        return torch.from_numpy(mttkrp(_to_numpy(tensor), [_to_numpy(f) for f in factors], mode))
     if _is_torch(tensor):
         import torch
         X = tensor.contiguous()
-        if X.dtype != torch.float32 or not X.is_cuda:
-            raise TypeError("device mttkrp expects a float32 CUDA tensor")
-        F = [f.contiguous() for f in factors]
+        if X.dtype != torch.float32:
+            raise TypeError("device mttkrp expects a float32 CUDA tensor (the library computes in f32)")
+        F = [_dev_f32(f, X.device) for f in factors]
         out = torch.empty((shape[mode], R), dtype=torch.float32, device=X.device)
         ctx = _lib.torch_context(X)
         ptrs = (C.c_void_p * len(F))(*[f.data_ptr() for f in F])
@@ -72,7 +98,7 @@ def mttkrp(tensor, factors, mode: int):
                                       out.data_ptr()))
         return out
     X = _np32(tensor)
-    F = [_np32(f) for f in factors]
+    F = [_np32(_to_numpy(f)) for f in factors]
     out = np.empty((shape[mode], R), dtype=np.float32)
     ctx = _lib.context_for(0)
     ptrs = (C.c_void_p * len(F))(*[f.ctypes.data for f in F])
@@ -117,6 +143,8 @@ def to_bf16(tensor):
     """Round a float32 CUDA tensor to bfloat16 (RNE) with the library's conversion kernel."""
     import torch
     X = tensor.contiguous()
+    if not _is_cuda(X) or X.dtype != torch.float32:
+        raise TypeError("to_bf16 expects a float32 CUDA tensor")
     out = torch.empty(X.shape, dtype=torch.bfloat16, device=X.device)
     ctx = _lib.torch_context(X)
     check(lib().tenrso_convert_f32_to_bf16(ctx.ptr, X.data_ptr(), X.numel(), out.data_ptr()))
@@ -132,9 +160,9 @@ def mttkrp_bf16(tensor_bf16, factors, mode: int):
     shape = tuple(int(s) for s in tensor_bf16.shape)
     R = _check_mttkrp(shape, factors, mode)
     X = tensor_bf16.contiguous()
-    if X.dtype != torch.bfloat16:
-        raise TypeError("mttkrp_bf16 expects a bfloat16 tensor (see to_bf16)")
-    F = [f.contiguous().float() for f in factors]
+    if not _is_cuda(X) or X.dtype != torch.bfloat16:
+        raise TypeError("mttkrp_bf16 expects a bfloat16 CUDA tensor (see to_bf16)")
+    F = [_dev_f32(f, X.device) for f in factors]
     out = torch.empty((shape[mode], R), dtype=torch.float32, device=X.device)
     ctx = _lib.torch_context(X)
     ptrs = (C.c_void_p * len(F))(*[f.data_ptr() for f in F])
@@ -153,14 +181,18 @@ def khatri_rao(a, b):
         raise ValueError(f"Number of columns must match: A has {a.shape[1]} columns, B has {b.shape[1]} columns")
     I, K = int(a.shape[0]), int(a.shape[1])
     J = int(b.shape[0])
+    if _is_torch(a) and not a.is_cuda:
+        import torch
+        return torch.from_numpy(khatri_rao(_to_numpy(a), _to_numpy(b)))
     if _is_torch(a):
         import torch
-        A, B = a.contiguous().float(), b.contiguous().float()
+        A = _dev_f32(a, a.device)
+        B = _dev_f32(b, a.device)
         out = torch.empty((I * J, K), dtype=torch.float32, device=A.device)
         ctx = _lib.torch_context(A)
         check(lib().tenrso_khatri_rao_f32(ctx.ptr, A.data_ptr(), I, B.data_ptr(), J, K, out.data_ptr()))
         return out
-    A, B = _np32(a), _np32(b)
+    A, B = _np32(a), _np32(_to_numpy(b))
     out = np.empty((I * J, K), dtype=np.float32)
     if out.size:
         ctx = _lib.context_for(0)
@@ -198,17 +230,19 @@ def cp_reconstruct(factors, weights=None):
     if weights is not None and len(weights) != R:
         raise ValueError(f"Weights length {len(weights)} must match rank {R}")
     shape = [int(f.shape[0]) for f in factors]
-    if _is_torch(factors[0]):
+    if _is_cuda(factors[0]):
         import torch
-        F = [f.contiguous().float() for f in factors]
-        w = None if weights is None else weights.contiguous().float()
-        out = torch.empty(shape, dtype=torch.float32, device=F[0].device)
+        device = factors[0].device
+        F = [_dev_f32(f, device) for f in factors]
+        w = None if weights is None else _dev_f32(weights, device)
+        out = torch.empty(shape, dtype=torch.float32, device=device)
         ctx = _lib.torch_context(F[0])
         ptrs = (C.c_void_p * len(F))(*[f.data_ptr() for f in F])
         check(lib().tenrso_cp_reconstruct_f32(ctx.ptr, len(F), _i64s(shape), ptrs, R,
                                               None if w is None else w.data_ptr(), out.data_ptr()))
         return out
     import torch
-    F = [torch.from_numpy(_np32(f)).cuda() for f in factors]
-    w = None if weights is None else torch.from_numpy(_np32(weights)).cuda()
-    return cp_reconstruct(F, w).cpu().numpy()
+    F = [_dev_f32(f, "cuda") for f in factors]
+    w = None if weights is None else _dev_f32(weights, "cuda")
+    out = cp_reconstruct(F, w).cpu()
+    return out if _is_torch(factors[0]) else out.numpy()

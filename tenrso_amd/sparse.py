@@ -350,7 +350,11 @@ class SparseTensor:
     def mttkrp(self, factors, mode):
         """Sparse MTTKRP; factors are float32 CUDA tensors, returns a CUDA tensor."""
         import torch
-        F = [f.contiguous().float() for f in factors]
+        from .kernels import _dev_f32
+        # factors of any float dtype or device are converted to float32 on the
+        # device the tensor lives on (never reinterpreted)
+        dev = torch.device("cuda", self.ctx.device)
+        F = [_dev_f32(f, dev) for f in factors]
         if len(F) != len(self.shape):
             raise ValueError(f"Number of factor matrices ({len(F)}) must match tensor rank ({len(self.shape)})")
         R = int(F[0].shape[1])

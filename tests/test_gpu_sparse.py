@@ -91,6 +91,21 @@ def test_sparse_mttkrp_vs_fp64(shape, nnz, R):
         assert rel(got, oracle.mttkrp_fast(dense, F64, mode)) <= 1e-5
 
 
+def test_sparse_mttkrp_converts_factors():
+    # float64 CUDA, torch CPU and numpy factors are converted to float32 on the
+    # tensor's device before the launch (never passed through as raw pointers)
+    shape, nnz, R = (30, 20, 10), 1500, 8
+    idx, vals = random_coo(shape, nnz, seed=77)
+    sp = SparseTensor(idx, vals, shape)
+    rng = np.random.default_rng(78)
+    F = [rng.standard_normal((n, R)) for n in shape]
+    for mode in range(3):
+        ref = oracle.coo_mttkrp(idx, vals, shape, F, mode)
+        for fs in ([torch.from_numpy(f).cuda() for f in F], [torch.from_numpy(f) for f in F], F):
+            got = sp.mttkrp(fs, mode)
+            assert got.is_cuda and rel(got.cpu().numpy(), ref) <= 1e-5
+
+
 def test_sparse_skewed_rows_and_edge_cases():
     # one row holding most nonzeros spans many worker chunks; plus an empty tensor,
     # a single nonzero and duplicate coordinates (summed)

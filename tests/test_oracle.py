@@ -149,13 +149,62 @@ def test_solve_ls_normal_equations(orc):
     assert np.allclose(Xs, ref, rtol=1e-9, atol=1e-9)
 
 
-def test_solve_ls_singular_uses_fallback(orc):
-    # rank-deficient Gram (zero factor column): Cholesky fails -> ridge retry -> pinv
+def test_solve_ls_singular_takes_ridge_retry(orc):
+    # rank-deficient Gram (zero factor column): Cholesky fails -> ridge retry with
+    # Gamma + eps 10R I (helpers.rs:477-490).  The closed form of the diagonal
+    # system pins eps: x_k = M_k / (G_kk + eps 10R), for the f32 epsilon (the
+    # instantiation the device replaces) and for the f64 one.
     G = np.diag([4.0, 1.0, 0.0])
-    M = np.array([[8.0, 3.0, 0.0]])
-    Xs, path = orc.solve_ls(M, G)
-    assert path >= 1
-    assert np.allclose(Xs[0, :2], [2.0, 3.0], rtol=1e-6)
+    M = np.array([[8.0, 3.0, 1e-6]])
+    for eps in (np.finfo(np.float32).eps, np.finfo(np.float64).eps):
+        Xs, path = orc.solve_ls(M, G, ridge_eps=float(eps))
+        assert path == 1
+        ridge = float(eps) * 30.0
+        assert np.allclose(Xs[0], [8.0 / (4.0 + ridge), 3.0 / (1.0 + ridge), 1e-6 / ridge], rtol=1e-12)
+    Xd, _ = orc.solve_ls(M, G)  # default: the FP64 epsilon
+    assert np.isclose(Xd[0, 2], 1e-6 / (np.finfo(np.float64).eps * 30.0), rtol=1e-12)
+
+
+def test_solve_ls_linalg_error_after_failed_retry(orc):
+    # a system that fails the ridge retry too is CpError::LinalgError
+    # (helpers.rs:485): no pseudo-inverse fallback.  A non-finite Gram fails both
+    # factorisations, as does an indefinite one whose negative eigenvalue exceeds
+    # the ridge.
+    M = np.ones((2, 3))
+    for G in (np.diag([1.0, np.nan, 1.0]), np.diag([1.0, -1.0, 2.0]), np.full((3, 3), np.inf)):
+        with pytest.raises(orc.LinalgError):
+            orc.solve_ls(M, G)
+
+
+def test_cp_als_linalg_error(orc):
+    # cp_als propagates the solve's LinalgError (core.rs:142 `?`): an infinite
+    # entry in an initial factor makes every Gamma_n non-finite.
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((6, 5, 4)).astype(np.float32)
+    init = [rng.standard_normal((n, 3)).astype(np.float32) for n in X.shape]
+    init[2][1, 0] = np.inf
+    with pytest.raises(orc.LinalgError):
+        orc.cp_als(X, init, 5, 0.0)
+
+
+def test_cp_als_oscillation_rule(orc):
+    # core.rs:149-164: every fit decrease counts; the run stops with Oscillation once
+    # more than 5 decreases have been counted after iteration 10.  An orthogonally
+    # constrained fit of a noise tensor oscillates; the rule is checked against
+    # the fit history itself.
+    rng = np.random.default_rng(126)
+    shp = tuple(int(x) for x in rng.integers(8, 24, size=3))
+    R = int(rng.integers(2, 6))
+    X = rng.standard_normal(shp).astype(np.float32)
+    init = [rng.standard_normal((n, R)).astype(np.float32) for n in shp]
+    _, info = orc.cp_als(X, init, 40, 0.0, orthogonal=True)
+    assert info["reason"] == "Oscillation"
+    h = info["fit_history"]
+    drops = np.diff(h) < 0
+    assert int(drops.sum()) == info["oscillation_count"] == 6
+    it = info["iters"] - 1
+    assert it > 10 and int(drops[:it - 1].sum()) <= 5   # not stopped earlier
+    assert info["oscillated"]
 
 
 def test_tikhonov_and_soft_threshold_kats(orc):
@@ -187,6 +236,28 @@ def test_fit_matches_dense_definition(orc):
     exp2 = 1 - np.linalg.norm(X2 - Xh) / np.linalg.norm(X2)
     assert abs(orc.fit(X2, F) - exp2) < 1e-10
     assert abs(orc.fit(X2, F, fast=False) - exp2) < 1e-10
+
+
+def test_cp_als_x_lead_model(orc):
+    # x_lead = X reproduces the plain run bit for bit; with another tensor Y the
+    # leading modes solve A_n Gamma_n = mttkrp(Y, ., n), the last mode
+    # A_{N-1} Gamma = mttkrp(X, ., N-1), and the fit is the fit of the result to X.
+    rng = np.random.default_rng(14)
+    X = rng.standard_normal((7, 6, 5))
+    Y = X + 1e-2 * rng.standard_normal(X.shape)
+    init = [rng.standard_normal((n, 3)) for n in X.shape]
+    Fa, ia = orc.cp_als(X, init, 4, 0.0)
+    Fb, ib = orc.cp_als(X, init, 4, 0.0, x_lead=X)
+    assert all(np.array_equal(a, b) for a, b in zip(Fa, Fb)) and ia["fit"] == ib["fit"]
+    F1, i1 = orc.cp_als(X, init, 1, 0.0, x_lead=Y)
+    cur = [f.copy() for f in init]
+    for n in range(3):
+        G = orc.gram_hadamard(cur, n)
+        M = orc.mttkrp_fast(Y if n < 2 else X, cur, n)
+        assert np.allclose(F1[n] @ G, M, rtol=1e-9, atol=1e-9)
+        cur[n] = F1[n]
+    assert abs(i1["fit"] - orc.fit(X, F1)) < 1e-12
+    assert abs(i1["fit"] - orc.fit(Y, F1)) > 1e-6
 
 
 def test_cp_als_exact_low_rank_recovery(orc):
