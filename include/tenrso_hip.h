@@ -140,16 +140,26 @@ int tenrso_khatri_rao_f32_host(tenrso_ctx* ctx, const float* A, int64_t I, const
  * ------------------------------------------------------------------------ */
 int tenrso_cp_reconstruct_f32(tenrso_ctx* ctx, int ndim, const int64_t* shape, const float* const* factors,
                               int64_t rank, const float* weights, float* out);
+/* CpDecomp::extract_weights (cp/types.rs:421-446), in place on device factors:
+ * every column r of every factor with norm n > FLT_EPSILON is divided by n, and
+ * weights[r] (device, rank floats) = product of those norms (1 where none is).
+ * Norms are summed in FP64 and rounded to fp32 for the division. */
+int tenrso_cp_extract_weights_f32(tenrso_ctx* ctx, int ndim, const int64_t* shape, float* const* factors,
+                                  int64_t rank, float* weights);
 
 /* ------------------------------------------------------------------------
  * CP-ALS.  Replaces tenrso_decomp::cp_als (cp/core.rs:59-182) and
  * cp_als_constrained (cp/core.rs:225-408) for T = f32, with the initial
  * factors supplied by the caller (the reference draws them from an unseeded
  * thread_rng, cp/helpers.rs:212-231; a shim draws them and passes them here).
- * Per iteration: for each mode, MTTKRP -> Gram-Hadamard (FP64) -> Cholesky
- * solve (FP64, ridge retry, eigen pseudo-inverse fallback) -> constraints; then
- * the fit (FP64).  Convergence, oscillation and time-limit rules are the
- * reference's (core.rs:117-165).
+ * Per iteration: for each mode, MTTKRP -> Gram-Hadamard (FP64) -> inverse of
+ * Gamma (FP64 block Gauss-Jordan, positive-definiteness tested by Sylvester's
+ * criterion) -> row solve -> constraints; then the fit (FP64).  A Gamma that is
+ * not positive definite is retried once with Gamma += FLT_EPSILON * 10R * I
+ * (the ridge retry of cp/helpers.rs:477-490); if that fails too the call returns
+ * TENRSO_ERR_LINALG (CpError::LinalgError) and the factor buffers hold the values
+ * of an unfinished sweep (undefined).  Convergence, oscillation and time-limit
+ * rules are the reference's (core.rs:117-165).
  * ------------------------------------------------------------------------ */
 typedef struct tenrso_cp_options {
   int64_t rank;
@@ -178,7 +188,7 @@ typedef struct tenrso_cp_result {
   int32_t oscillated;
   double fit;
   double final_fit_change;
-  int32_t solve_path_max;        /* 0 Cholesky, 1 ridge retry, 2 pseudo-inverse */
+  int32_t solve_path_max;        /* 0 direct solve, 1 ridge retry was needed */
   int32_t reserved;
   double seconds;                /* wall time of the ALS loop */
 } tenrso_cp_result;
@@ -218,7 +228,8 @@ int tenrso_mttkrp_vjp_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64
                           const float* const* factors, int64_t rank, int mode, const float* cotangent,
                           float* const* dfactors, float* dX);
 /* CpReconstructionGrad::compute_factor_gradients (tenrso-ad/src/grad.rs:81-120):
- * grads[n] = mttkrp(grad_output, factors, n) * diag(weights) (weights may be NULL). */
+ * grads[n] = mttkrp(grad_output, factors, n) * diag(weights) (weights may be NULL;
+ * a NULL grads[n] skips mode n). */
 int tenrso_cp_factor_grads_f32(tenrso_ctx* ctx, const float* grad_output, int ndim, const int64_t* shape,
                                const float* const* factors, int64_t rank, const float* weights,
                                float* const* grads);
@@ -227,6 +238,37 @@ int tenrso_cp_factor_grads_f32(tenrso_ctx* ctx, const float* grad_output, int nd
  * grads may be NULL (loss only). */
 int tenrso_cp_loss_grad_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
                             const float* const* factors, int64_t rank, float* const* grads, double* loss);
+/* AdOperation-shaped rules (tenrso-ad/src/hooks.rs:44-74: forward, backward, name,
+ * num_inputs, num_outputs; a VjpOp, tenrso-ad/src/vjp.rs:30-44, is backward with
+ * one output gradient), over BORROWED device inputs that must outlive the op:
+ *   TENRSO_AD_MTTKRP          inputs [X, A_0..A_{N-1}], output [M] (shape[mode] x rank);
+ *                             backward: dX, dA_m (m != mode) as tenrso_mttkrp_vjp_f32,
+ *                             dA_mode = 0
+ *   TENRSO_AD_CP_RECONSTRUCT  inputs [A_0..A_{N-1}] (X unused), weights optional,
+ *                             output [Xhat] (prod(shape)); backward: dA_n =
+ *                             mttkrp(G, A, n) diag(w) (grad.rs:81-120)
+ *   TENRSO_AD_CP_LOSS         inputs [X, A_0..A_{N-1}], output [L] (1 float,
+ *                             0.5 ||X - Xhat||^2, FP64 inside); backward with the
+ *                             device cotangent g: dX = g (X - Xhat), dA_n =
+ *                             g (A_n Gamma_n - mttkrp(X, A, n)); rank 1..64
+ * forward/backward take arrays of device buffers (outputs[k] of
+ * tenrso_ad_op_output_numel(op, k) floats; input_grads[k] of
+ * tenrso_ad_op_input_numel(op, k) floats).  A NULL input_grads array, or a NULL
+ * entry, skips that gradient.  Calls are asynchronous on the op's context stream. */
+enum tenrso_ad_kind { TENRSO_AD_MTTKRP = 0, TENRSO_AD_CP_RECONSTRUCT = 1, TENRSO_AD_CP_LOSS = 2 };
+typedef struct tenrso_ad_op tenrso_ad_op;
+int tenrso_ad_op_create(tenrso_ctx* ctx, int kind, const float* X, int ndim, const int64_t* shape,
+                        const float* const* factors, int64_t rank, int mode, const float* weights,
+                        tenrso_ad_op** out);
+int tenrso_ad_op_destroy(tenrso_ad_op* op);
+const char* tenrso_ad_op_name(const tenrso_ad_op* op);
+int tenrso_ad_op_num_inputs(const tenrso_ad_op* op);
+int tenrso_ad_op_num_outputs(const tenrso_ad_op* op);
+int64_t tenrso_ad_op_input_numel(const tenrso_ad_op* op, int k);
+int64_t tenrso_ad_op_output_numel(const tenrso_ad_op* op, int k);
+int tenrso_ad_op_forward(tenrso_ad_op* op, float* const* outputs);
+int tenrso_ad_op_backward(tenrso_ad_op* op, const float* const* output_grads, float* const* input_grads);
+
 /* Gradient-descent CP fit: `steps` simultaneous updates A_n -= lr * grad_n (lr <= 0:
  * lr = 1 / max_n ||Gamma_n||_F at each step).  loss_history[s] (host, may be NULL)
  * is the loss at the point step s starts from. */

@@ -647,14 +647,15 @@ void orc_cp_reconstruct_f64(const double* const* F, int nd, const int64_t* shape
 }
 
 // extract_weights (cp/types.rs:421-446): normalise columns, accumulate norms.
-void orc_extract_weights_f64(double** F, int nd, const int64_t* shape, int64_t R, double* w) {
+// eps is T::epsilon() of the instantiation being modelled (:433).
+void orc_extract_weights_f64(double** F, int nd, const int64_t* shape, int64_t R, double* w, double eps) {
   for (i64 r = 0; r < R; ++r) w[r] = 1.0;
   for (int k = 0; k < nd; ++k)
     for (i64 r = 0; r < R; ++r) {
       double s = 0.0;
       for (i64 i = 0; i < shape[k]; ++i) s = s + F[k][i * R + r] * F[k][i * R + r];
       double n = std::sqrt(s);
-      if (n > std::numeric_limits<double>::epsilon()) {
+      if (n > eps) {
         w[r] = w[r] * n;
         for (i64 i = 0; i < shape[k]; ++i) F[k][i * R + r] = F[k][i * R + r] / n;
       }

@@ -265,13 +265,16 @@ def cp_reconstruct(factors, weights=None):
     return out
 
 
-def extract_weights(factors):
-    """CpDecomp::extract_weights (cp/types.rs:421-446); returns (factors, weights)."""
+def extract_weights(factors, eps=float(np.finfo(np.float32).eps)):
+    """CpDecomp::extract_weights (cp/types.rs:421-446); returns (factors, weights).
+
+    FP64 arithmetic; eps is the T::epsilon() threshold of the instantiation being
+    modelled (:433): f32's by default (the instantiation the device replaces)."""
     F = [np.array(f, dtype=np.float64, order="C") for f in factors]
     shape = [f.shape[0] for f in F]
     R = F[0].shape[1]
     w = np.zeros(R)
-    lib().orc_extract_weights_f64(_ptrs(F), len(F), _shape(shape), C.c_int64(R), _p(w))
+    lib().orc_extract_weights_f64(_ptrs(F), len(F), _shape(shape), C.c_int64(R), _p(w), C.c_double(eps))
     return F, w
 
 

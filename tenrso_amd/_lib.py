@@ -26,6 +26,7 @@ ERR_UNSUPPORTED = 9
 PREC_FP32 = 0
 PREC_BF16 = 1
 INIT_SVD, INIT_NNSVD, INIT_LEVERAGE = 2, 3, 4  # enum tenrso_init_kind
+AD_MTTKRP, AD_CP_RECONSTRUCT, AD_CP_LOSS = 0, 1, 2  # enum tenrso_ad_kind
 
 REASONS = ["FitTolerance", "MaxIterations", "Oscillation", "TimeLimit"]
 
@@ -65,6 +66,7 @@ SIGNATURES = {
     "tenrso_khatri_rao_f32": ([_vp, _vp, _i64, _vp, _i64, _i64, _vp], _i),
     "tenrso_khatri_rao_f32_host": ([_vp, _vp, _i64, _vp, _i64, _i64, _vp], _i),
     "tenrso_cp_reconstruct_f32": ([_vp, _i, _vp, _vp, _i64, _vp, _vp], _i),
+    "tenrso_cp_extract_weights_f32": ([_vp, _i, _vp, _vp, _i64, _vp], _i),
     "tenrso_cp_als_f32": ([_vp, _vp, _i, _vp, _vp, C.POINTER(CpOptions), C.POINTER(CpResult), _vp], _i),
     "tenrso_cp_als_f32_host": ([_vp, _vp, _i, _vp, _vp, C.POINTER(CpOptions), C.POINTER(CpResult), _vp], _i),
     "tenrso_cp_init_svd": ([_vp, _vp, _i, _vp, _i64, _i, C.c_uint64, _vp], _i),
@@ -72,6 +74,15 @@ SIGNATURES = {
     "tenrso_cp_factor_grads_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _vp, _vp], _i),
     "tenrso_cp_loss_grad_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _vp, C.POINTER(_d)], _i),
     "tenrso_cp_gd_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _i, _d, _vp], _i),
+    "tenrso_ad_op_create": ([_vp, _i, _vp, _i, _vp, _vp, _i64, _i, _vp, C.POINTER(_vp)], _i),
+    "tenrso_ad_op_destroy": ([_vp], _i),
+    "tenrso_ad_op_name": ([_vp], C.c_char_p),
+    "tenrso_ad_op_num_inputs": ([_vp], _i),
+    "tenrso_ad_op_num_outputs": ([_vp], _i),
+    "tenrso_ad_op_input_numel": ([_vp, _i], _i64),
+    "tenrso_ad_op_output_numel": ([_vp, _i], _i64),
+    "tenrso_ad_op_forward": ([_vp, _vp], _i),
+    "tenrso_ad_op_backward": ([_vp, _vp, _vp], _i),
     "tenrso_sp_create": ([_vp, _i, _vp, _i64, _vp, _vp, _i, C.POINTER(_vp)], _i),
     "tenrso_sp_destroy": ([_vp], _i),
     "tenrso_sp_mode_arrays": ([_vp, _vp, _i, _vp, _vp, _vp, _vp], _i),

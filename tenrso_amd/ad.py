@@ -4,8 +4,10 @@
   X^ = sum_r w_r a_1^r o ... o a_N^r, i.e. dA_n = mttkrp(dL/dX^, A, n) diag(w).
 * ``MttkrpVjp``: VjpOp-shaped rule (tenrso-ad/src/vjp.rs:30-44) for
   M = mttkrp(X, A, mode); ``vjp(Mbar)`` returns [dX] + [dA_m for m != mode].
-* ``CpLoss``: AdOperation-shaped rule (tenrso-ad/src/hooks.rs:44-74) for
-  L = 0.5 ||X - [[A]]||^2 with gradients A_n Gamma_n - mttkrp(X, A, n).
+* ``DeviceAdOperation``: AdOperation-shaped rules (tenrso-ad/src/hooks.rs:44-74)
+  held by the library (``tenrso_ad_op_*``): mttkrp, cp_reconstruct, cp_loss.
+* ``CpLoss``: the cp_loss rule, L = 0.5 ||X - [[A]]||^2, gradients
+  X - Xhat and A_n Gamma_n - mttkrp(X, A, n).
 * ``cp_gd``: gradient-descent CP fit on the device.
 * ``check_gradient`` / ``GradCheckConfig`` (tenrso-ad/src/gradcheck.rs:43-268):
   the reference's element-wise central finite-difference checker (host logic,
@@ -25,8 +27,8 @@ from . import _lib
 from ._lib import check, lib
 from .kernels import _check_mttkrp, _i64s, mttkrp
 
-__all__ = ["CpReconstructionGrad", "MttkrpVjp", "CpLoss", "cp_loss_and_grad", "cp_gd", "GradCheckConfig",
-           "GradCheckResult", "check_gradient", "AdContext"]
+__all__ = ["CpReconstructionGrad", "MttkrpVjp", "CpLoss", "DeviceAdOperation", "cp_loss_and_grad", "cp_gd",
+           "GradCheckConfig", "GradCheckResult", "check_gradient", "AdContext"]
 
 
 def _dev(t, what):
@@ -134,33 +136,95 @@ def cp_loss_and_grad(tensor, factors, with_grad: bool = True):
     return loss.value, grads
 
 
-class CpLoss:
-    """AdOperation-shaped (hooks.rs:44-74) rule for L(A_1..A_N) = 0.5 ||X - [[A]]||^2."""
+class DeviceAdOperation:
+    """AdOperation-shaped rule (hooks.rs:44-74) held by the HIP library
+    (``tenrso_ad_op_*``): ``forward()``, ``backward(output_grads, needs=None)``,
+    ``vjp(output_grad)`` (the VjpOp shape, vjp.rs:30-44), ``name()``,
+    ``num_inputs()``, ``num_outputs()``.  The inputs are borrowed by the library,
+    so this object keeps them alive.  ``needs`` (one bool per input) limits which
+    gradients are computed; the others come back as None."""
 
-    def __init__(self, tensor, factors):
-        self.X = _dev(tensor, "tensor")
-        self.factors = [_dev(f, "factor") for f in factors]
-        self._grads = None
-
-    def forward(self):
+    def __init__(self, kind, tensor, factors, mode=0, weights=None):
         import torch
-        loss, self._grads = cp_loss_and_grad(self.X, self.factors, True)
-        return [torch.tensor(loss, dtype=torch.float64)]
+        self.factors = [_dev(f, "factor") for f in factors]
+        self.X = None if tensor is None else _dev(tensor, "tensor")
+        self.weights = None if weights is None else _dev(weights, "weights")
+        if self.X is not None:
+            self.shape = tuple(int(s) for s in self.X.shape)
+            self.device = self.X.device
+        else:
+            self.shape = tuple(int(f.shape[0]) for f in self.factors)
+            self.device = self.factors[0].device
+        self.R = _check_mttkrp(self.shape, self.factors, int(mode) if kind == _lib.AD_MTTKRP else 0)
+        self._ctx = _lib.torch_context(self.factors[0])
+        self._ptr = C.c_void_p()
+        check(lib().tenrso_ad_op_create(self._ctx.ptr, int(kind), None if self.X is None else self.X.data_ptr(),
+                                        len(self.shape), _i64s(self.shape), _ptrs(self.factors), self.R, int(mode),
+                                        None if self.weights is None else self.weights.data_ptr(),
+                                        C.byref(self._ptr)))
+        self.kind = int(kind)
+        self.mode = int(mode)
+        self._torch = torch
 
-    def backward(self, output_grads):
-        if self._grads is None:
-            self.forward()
-        g = float(output_grads[0]) if output_grads else 1.0
-        return [gr * g for gr in self._grads]
+    def _input_shape(self, k):
+        has_x = self.kind != _lib.AD_CP_RECONSTRUCT
+        if has_x and k == 0:
+            return self.shape
+        m = k - 1 if has_x else k
+        return (self.shape[m], self.R)
+
+    def _output_shape(self):
+        if self.kind == _lib.AD_MTTKRP:
+            return (self.shape[self.mode], self.R)
+        if self.kind == _lib.AD_CP_RECONSTRUCT:
+            return self.shape
+        return (1,)
 
     def name(self):
-        return "cp_loss"
+        return lib().tenrso_ad_op_name(self._ptr).decode()
 
     def num_inputs(self):
-        return len(self.factors)
+        return int(lib().tenrso_ad_op_num_inputs(self._ptr))
 
     def num_outputs(self):
-        return 1
+        return int(lib().tenrso_ad_op_num_outputs(self._ptr))
+
+    def forward(self):
+        out = self._torch.empty(self._output_shape(), dtype=self._torch.float32, device=self.device)
+        check(lib().tenrso_ad_op_forward(self._ptr, _ptrs([out])))
+        return [out]
+
+    def backward(self, output_grads, needs=None):
+        g = output_grads[0]
+        if not (isinstance(g, self._torch.Tensor) and g.is_cuda):
+            g = self._torch.tensor(g, dtype=self._torch.float32, device=self.device)
+        g = _dev(g.reshape(self._output_shape()).to(self._torch.float32), "output gradient")
+        n_in = self.num_inputs()
+        needs = [True] * n_in if needs is None else list(needs)
+        grads = [self._torch.empty(self._input_shape(k), dtype=self._torch.float32, device=self.device)
+                 if needs[k] else None for k in range(n_in)]
+        check(lib().tenrso_ad_op_backward(self._ptr, _ptrs([g]), _ptrs(grads)))
+        return grads
+
+    def vjp(self, output_grad):
+        return self.backward([output_grad])
+
+    def __del__(self):
+        try:
+            if self._ptr:
+                lib().tenrso_ad_op_destroy(self._ptr)
+                self._ptr = C.c_void_p()
+        except Exception:
+            pass
+
+
+class CpLoss(DeviceAdOperation):
+    """AdOperation-shaped (hooks.rs:44-74) rule for L = 0.5 ||X - [[A_1..A_N]]||^2:
+    inputs [X, A_1..A_N], one output (the loss); backward(g) returns
+    [g (X - Xhat), g (A_n Gamma_n - mttkrp(X, A, n)) for each n], on the device."""
+
+    def __init__(self, tensor, factors):
+        super().__init__(_lib.AD_CP_LOSS, tensor, factors)
 
 
 class AdContext:

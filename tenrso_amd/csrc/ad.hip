@@ -210,6 +210,7 @@ int mttkrp_vjp(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, const 
 int cp_factor_grads(tenrso_ctx* ctx, const float* Gt, int nd, const i64* shape, const float* const* F, i64 R,
                     const float* w, float* const* grads) {
   for (int n = 0; n < nd; ++n) {
+    if (!grads[n]) continue;
     TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, Gt, nullptr, nd, shape, F, R, n, MttkrpOut::F32, grads[n],
                             nullptr));
     if (w) {
@@ -276,8 +277,8 @@ static int loss_grad_device(tenrso_ctx* ctx, const float* X, int nd, const i64* 
   return TENRSO_OK;
 }
 
-int cp_loss_grad(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, const float* const* F, i64 R,
-                 float* const* grads, double* loss_host) {
+int cp_loss_grad_dev(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, const float* const* F, i64 R,
+                     float* const* grads, const double** loss_dev) {
   if (R > SOLVE_RMAX) {
     set_error("rank %lld exceeds the on-device Gram bound %d", (long long)R, SOLVE_RMAX);
     return TENRSO_ERR_UNSUPPORTED;
@@ -289,8 +290,16 @@ int cp_loss_grad(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, cons
   TENRSO_TRY(fill_f64(ctx, w.scal, 64, 0.0));
   TENRSO_TRY(tensor_norm_min(ctx, X, total, w.part, w.scal));
   TENRSO_TRY(loss_grad_device(ctx, X, nd, shape, F, R, grads, w, nullptr));
+  if (loss_dev) *loss_dev = w.out;
+  return TENRSO_OK;
+}
+
+int cp_loss_grad(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, const float* const* F, i64 R,
+                 float* const* grads, double* loss_host) {
+  const double* out = nullptr;
+  TENRSO_TRY(cp_loss_grad_dev(ctx, X, nd, shape, F, R, grads, &out));
   double* hs = (double*)ctx->pinned;
-  TENRSO_HIP_TRY(hipMemcpyAsync(hs, w.out, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  TENRSO_HIP_TRY(hipMemcpyAsync(hs, out, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
   if (loss_host) *loss_host = hs[0];
   return TENRSO_OK;

@@ -312,6 +312,31 @@ int tenrso_cp_reconstruct_f32(tenrso_ctx* ctx, int ndim, const int64_t* shape, c
   return cp_reconstruct_f32(ctx, ndim, shape, factors, rank, weights, out);
 }
 
+int tenrso_cp_extract_weights_f32(tenrso_ctx* ctx, int ndim, const int64_t* shape, float* const* factors,
+                                  int64_t rank, float* weights) {
+  clear_error();
+  if (bad_ctx(ctx)) return TENRSO_ERR_INVALID_ARGUMENT;
+  if (ndim <= 0 || !shape || !factors || !weights) {
+    set_error("Need at least one factor matrix and a weights buffer to extract weights");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  if (ndim > MAXD_RECON) {
+    set_error("tensors with more than %d modes are not supported", MAXD_RECON);
+    return TENRSO_ERR_UNSUPPORTED;
+  }
+  if (rank < 0) {
+    set_error("negative rank %lld", (long long)rank);
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  for (int k = 0; k < ndim; ++k)
+    if (shape[k] < 0 || (shape[k] > 0 && rank > 0 && !factors[k])) {
+      set_error("factor %d: negative size or null pointer", k);
+      return TENRSO_ERR_INVALID_ARGUMENT;
+    }
+  TENRSO_TRY(set_device(ctx));
+  return cp_extract_weights_f32(ctx, ndim, shape, factors, rank, weights);
+}
+
 int tenrso_cp_als_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, float* const* factors,
                       const tenrso_cp_options* opts, tenrso_cp_result* result, double* fit_history) {
   clear_error();

@@ -211,25 +211,11 @@ class CpDecomp:
         return cp_reconstruct(self.factors, self.weights)
 
     def extract_weights(self):
-        """CpDecomp::extract_weights (types.rs:421-446): unit-norm columns, weights = product of norms."""
-        eps = np.finfo(np.float32).eps
-        R = int(self.factors[0].shape[1])
-        if type(self.factors[0]).__module__.startswith("torch"):
-            import torch
-            w = torch.ones(R, dtype=self.factors[0].dtype, device=self.factors[0].device)
-            for k, f in enumerate(self.factors):
-                n = torch.sqrt((f * f).sum(0))
-                keep = n > eps
-                w = torch.where(keep, w * n, w)
-                self.factors[k] = torch.where(keep[None, :], f / torch.where(keep, n, torch.ones_like(n)), f)
-        else:
-            w = np.ones(R, dtype=np.float32)
-            for k, f in enumerate(self.factors):
-                n = np.sqrt((f.astype(np.float32) ** 2).sum(0))
-                keep = n > eps
-                w = np.where(keep, w * n, w).astype(np.float32)
-                self.factors[k] = np.where(keep[None, :], f / np.where(keep, n, 1.0), f).astype(np.float32)
-        self.weights = w
+        """CpDecomp::extract_weights (types.rs:421-446): unit-norm columns, weights =
+        product of the column norms, computed by the device kernel
+        (tenrso_cp_extract_weights_f32)."""
+        from .kernels import extract_weights
+        self.factors, self.weights = extract_weights(self.factors)
 
 
 # ---------------------------------------------------------------- entry points

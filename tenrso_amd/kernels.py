@@ -23,7 +23,7 @@ from ._lib import check, lib
 __all__ = [
     "mttkrp", "mttkrp_blocked", "mttkrp_blocked_parallel", "mttkrp_fused", "mttkrp_fused_parallel",
     "mttkrp_all_modes", "mttkrp_bf16", "to_bf16", "khatri_rao", "khatri_rao_parallel", "khatri_rao_blocked",
-    "khatri_rao_blocked_parallel", "cp_reconstruct",
+    "khatri_rao_blocked_parallel", "cp_reconstruct", "extract_weights",
 ]
 
 
@@ -246,3 +246,39 @@ def cp_reconstruct(factors, weights=None):
     w = None if weights is None else _dev_f32(weights, "cuda")
     out = cp_reconstruct(F, w).cpu()
     return out if _is_torch(factors[0]) else out.numpy()
+
+
+def extract_weights(factors):
+    """CpDecomp::extract_weights (cp/types.rs:421-446) on the device.
+
+    Returns (normalised factors, weights): every column with norm > f32 epsilon
+    divided by its norm, weights = product of the column norms over the factors.
+    CUDA float32 contiguous factors are normalised in place; other inputs are
+    copied to the device and the results come back in the input's kind (numpy or
+    torch CPU).
+    """
+    if len(factors) == 0:
+        raise ValueError("Need at least one factor matrix to extract weights")
+    R = int(factors[0].shape[1])
+    for i, f in enumerate(factors):
+        if len(f.shape) != 2 or int(f.shape[1]) != R:
+            raise ValueError(f"Factor {i} has {f.shape[1] if len(f.shape) == 2 else '?'} columns, expected {R}")
+    import torch
+    on_dev = _is_cuda(factors[0])
+    device = factors[0].device if on_dev else torch.device("cuda")
+    F = []
+    for f in factors:
+        if _is_cuda(f) and f.device == device and f.dtype == torch.float32 and f.is_contiguous():
+            F.append(f)
+        else:
+            F.append(_dev_f32(f, device))
+    w = torch.empty(R, dtype=torch.float32, device=device)
+    ctx = _lib.torch_context(F[0])
+    ptrs = (C.c_void_p * len(F))(*[f.data_ptr() for f in F])
+    check(lib().tenrso_cp_extract_weights_f32(ctx.ptr, len(F), _i64s([f.shape[0] for f in F]), ptrs, R,
+                                              w.data_ptr()))
+    if on_dev:
+        return F, w
+    if _is_torch(factors[0]):
+        return [f.cpu() for f in F], w.cpu()
+    return [f.cpu().numpy() for f in F], w.cpu().numpy()

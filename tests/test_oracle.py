@@ -343,6 +343,19 @@ def test_extract_weights(orc):
     assert np.allclose(orc.cp_reconstruct(Fn, w), orc.cp_reconstruct(F))
     for f in Fn:
         assert np.allclose(np.linalg.norm(f, axis=0), 1.0)
+    # by hand: columns (3, 4) and (0, 2) of A, (1, 0) and (0, 5) of B
+    A = np.array([[3.0, 0.0], [4.0, 2.0]])
+    B = np.array([[1.0, 0.0], [0.0, 5.0]])
+    Fn, w = orc.extract_weights([A, B])
+    assert np.array_equal(w, [5.0, 10.0])
+    assert np.array_equal(Fn[0], [[0.6, 0.0], [0.8, 1.0]]) and np.array_equal(Fn[1], [[1.0, 0.0], [0.0, 1.0]])
+    # the T::epsilon() threshold (types.rs:433): a column of norm 1e-9 is below
+    # f32's epsilon (kept as is, weight 1) and above f64's (normalised)
+    C_ = np.array([[1e-9, 2.0], [0.0, 0.0]])
+    Fn32, w32 = orc.extract_weights([C_])
+    Fn64, w64 = orc.extract_weights([C_], eps=float(np.finfo(np.float64).eps))
+    assert np.array_equal(w32, [1.0, 2.0]) and Fn32[0][0, 0] == 1e-9
+    assert np.allclose(w64, [1e-9, 2.0], rtol=1e-15) and Fn64[0][0, 0] == 1.0
 
 
 def test_cp_factor_grads_fd_and_identity(orc):
