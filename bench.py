@@ -54,6 +54,18 @@ def profile_kernel(ctx, prec, X, shape, F, R, mode, reps):
     return ms.value
 
 
+def _cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), else what platform reports."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(X, init, gpu_fits, max_iters):
     """CPU leg: the faithful f32 restatement of the reference cp_als (single thread),
     timed on a bounded sample of one ALS iteration of the same workload, plus the
@@ -79,13 +91,17 @@ def cpu_baseline(X, init, gpu_fits, max_iters):
         "kind": "port",
         "sample": "1 iteration of the faithful f32 restatement of cp_als (3 mode MTTKRPs with unfolding copy and "
                   "materialised Khatri-Rao + per-row solves + the fit's mode-0 MTTKRP) on the same 256^3 r64 tensor",
-        "cpu": platform.processor() or platform.machine(),
-        "host_threads_available": os.cpu_count(),
+        "cpu": _cpu_model(),
+        "host_threads_available": len(os.sched_getaffinity(0)),
         "oracle_fp64_fit": info64["fit"],
         "fit_delta": {k: abs(v - info64["fit"]) for k, v in gpu_fits.items()},
         "all_cores_mttkrp": {"s_per_mode": t_modes, "threads": oracle.num_threads(), "kind": "port",
                              "sample": "one mttkrp_blocked_parallel restatement (OpenMP, tile 64) per mode of the "
                                        "same tensor"},
+        # one ALS iteration on every host thread: the three mode MTTKRPs and the
+        # fit's mode-0 MTTKRP with the OpenMP restatement (the per-row solves, under
+        # 1 % of the faithful iteration, are left out)
+        "all_cores_s_per_iter": sum(t_modes) + t_modes[0],
     }
 
 
@@ -152,6 +168,25 @@ def extra_configs(with_cpu):
         c3[prec] = {"s_per_iter": ms / 1e4, "fit": cp.fit}
     out["congruent_512x512x128_r64"] = c3
     del Xd, Xb
+
+    # 250^3 rank 64: a ragged fastest mode (250 is no multiple of 32; the MFMA
+    # kernels mask the end of the last window), same model as the headline tensor
+    X, _, _, init = synth.dense_256_r64(shape=(250, 250, 250))
+    Xd = torch.from_numpy(X).to(dev)
+    Fd = [torch.from_numpy(f).to(dev) for f in init]
+    ctx = _lib.torch_context(Xd)
+    shape, R = X.shape, 64
+    modes = {}
+    for mode in range(3):
+        t32 = profile_kernel(ctx, _lib.PREC_FP32, Xd, shape, Fd, R, mode, 10)
+        modes[str(mode)] = {"fp32_us": t32 * 1e3, "fp32_TFLOPs": mttkrp_flops(shape, R) / (t32 * 1e-3) / 1e12}
+    cr = {"shape": list(shape), "rank": R, "mttkrp_per_mode": modes}
+    for prec in ("fp32", "bf16"):
+        decomp.cp_als_with_init(Xd, Fd, 2, 0.0, precision=prec)
+        cp, ms = _dev_timed(lambda: decomp.cp_als_with_init(Xd, Fd, 10, 0.0, precision=prec))
+        cr[prec] = {"s_per_iter": ms / 1e4, "fit": cp.fit}
+    out["dense_250_r64_ragged"] = cr
+    del Xd
 
     # 96^4 rank 32: MTTKRP per mode, CP-loss gradient, gradient-descent steps
     X, _, _, init = synth.dense4_96_r32()

@@ -87,33 +87,49 @@ struct FastWindow {
   floatx4 v[PERU][2];
 };
 
+// kv: rows of the window that exist (kw, or fewer in the last window of a
+// fastest mode that kw does not divide); a row past kv is read as row 0 (always
+// inside FAST) and stored as zero by stage_window_store*.
 template <bool WV, int RT, int NTH, int KWMAX>
-__device__ __forceinline__ void stage_window_load(FastWindow<RT, NTH, KWMAX>& fw, const KernArgs& a, int w) {
+__device__ __forceinline__ void stage_window_load(FastWindow<RT, NTH, KWMAX>& fw, const KernArgs& a, int w, int kv) {
   const float* base = a.fast + (i64)w * a.kw * a.R + a.r0;
   if constexpr (WV) {
 #pragma unroll
     for (int e = 0; e < FastWindow<RT, NTH, KWMAX>::PERU; ++e) {
       const int u = threadIdx.x + e * NTH;
       const int r4 = 4 * (u % (RT / 4)), kk = 2 * (u / (RT / 4));
-      const bool ok = kk < a.kw && r4 < a.rcount;
+      const bool ok = kk < kv && r4 < a.rcount;
+      const bool ok1 = ok && kk + 1 < kv;
       const float* p = base + (ok ? kk * a.R + r4 : 0);
       fw.v[e][0] = *reinterpret_cast<const floatx4*>(p);
-      fw.v[e][1] = *reinterpret_cast<const floatx4*>(p + (ok ? a.R : 0));
+      fw.v[e][1] = *reinterpret_cast<const floatx4*>(p + (ok1 ? a.R : 0));
     }
   } else {
 #pragma unroll
     for (int e = 0; e < FastWindow<RT, NTH, KWMAX>::PERU; ++e) {
       const int u = threadIdx.x + e * NTH;
       const int r4 = 4 * (u % (RT / 4)), kk = 2 * (u / (RT / 4));
-      const bool rows_ok = kk < a.kw;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const bool ok = rows_ok && r4 + q < a.rcount;
+        const bool ok = kk < kv && r4 + q < a.rcount;
+        const bool ok1 = ok && kk + 1 < kv;
         const float* p = base + (ok ? kk * a.R + r4 + q : 0);
         fw.v[e][0][q] = p[0];
-        fw.v[e][1][q] = p[ok ? a.R : 0];
+        fw.v[e][1][q] = p[ok1 ? a.R : 0];
       }
     }
+  }
+}
+
+// rows kk >= kv of a staged pair are zero (they meet masked, zero X elements)
+template <int RT, int NTH, int KWMAX>
+__device__ __forceinline__ void window_zero_tail(FastWindow<RT, NTH, KWMAX>& fw, int kv) {
+#pragma unroll
+  for (int e = 0; e < FastWindow<RT, NTH, KWMAX>::PERU; ++e) {
+    const int u = threadIdx.x + e * NTH;
+    const int kk = 2 * (u / (RT / 4));
+    if (kk >= kv) fw.v[e][0] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (kk + 1 >= kv) fw.v[e][1] = floatx4{0.f, 0.f, 0.f, 0.f};
   }
 }
 
@@ -338,7 +354,7 @@ __device__ __forceinline__ void fold_slow(floatx4 (&acc)[MT][NT], floatx4 (&T)[M
 //   are issued first, then the first three X blocks, so the window reaches LDS
 //   while X is still arriving.
 // ---------------------------------------------------------------------------
-template <int NT, int WAVES, int MT, bool CASE_B, int KWMAX, bool WV>
+template <int NT, int WAVES, int MT, bool CASE_B, int KWMAX, bool WV, int XL>
 __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __restrict__ X, KernArgs a,
                                                               float* __restrict__ partial, AlsJob job, int has_job) {
   constexpr int KB = 32, PITCH = KWMAX + 4, RT = 16 * NT, NTH = WAVES * 64, ROWS = 16 * MT;
@@ -366,7 +382,10 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
   // balanced split of the slow range: the first slow_rem splits take one more
   const i64 s_lo = uniform_i64((i64)ss * a.slow_q + (ss < a.slow_rem ? ss : a.slow_rem));
   const i64 s_hi = s_lo + a.slow_q + (ss < a.slow_rem ? 1 : 0);
-  const int nkb = a.kw / KB;
+  // rows of this window that exist: kw, or fewer in the last window of a fastest
+  // mode that kw does not divide (XL > 0: its loads are masked past kv)
+  const int kv = XL == 0 ? a.kw : (int)min((i64)a.kw, a.fast_size - (i64)w * a.kw);
+  const int nkb = XL == 0 ? a.kw / KB : (kv + KB - 1) / KB;
   // the workgroup's blocks (slow index, then block within it) in WAVES contiguous,
   // balanced ranges: with 9 slow indices of 4 blocks, 9 blocks per wave rather
   // than 12 for one wave and 8 for the others
@@ -377,7 +396,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
 
   // window loads first (see stage_window_load)
   FastWindow<RT, NTH, KWMAX> fw;
-  stage_window_load<WV>(fw, a, w);
+  stage_window_load<WV>(fw, a, w, kv);
 
   i64 rowc[MT];
   i64 rowv = 0;  // case B: first row of this lane's MT-vector
@@ -402,6 +421,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
     floatx4 x[MT][2];
     float pre[NT];
     int kb;
+    uint32_t okm;  // XL > 0: bit 4h + s set where k = kb*KB + 16h + 4g + s < kv
   };
 
   // per-lane element offsets, fixed for the whole loop; the per-block part of
@@ -434,12 +454,39 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
     // unconditional and costs the memory pipeline almost nothing
     const bool live = t < T;
     const float* ubase = X + (live ? cur.offset(a) : 0);
+    // XL > 0: elements at k >= kv are not loaded (their address becomes X) and
+    // are zeroed in compute(), where the value is used anyway (a select here
+    // would wait for the load and drain the prefetch)
+    uint32_t okm = 0xffu;
+    if constexpr (XL > 0) {
+      const int k0 = b.kb * KB + 4 * g;
+      okm = 0;
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int sidx = 0; sidx < 4; ++sidx)
+          okm |= (live && k0 + 16 * h + sidx < kv) ? 1u << (4 * h + sidx) : 0u;
+    }
+    b.okm = okm;
     if constexpr (!CASE_B) {
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         const float* base = ubase + (live ? loff[mt] : 0);
-        b.x[mt][0] = *reinterpret_cast<const floatx4*>(base);
-        b.x[mt][1] = *reinterpret_cast<const floatx4*>(base + 16);
+        if constexpr (XL == 0) {
+          b.x[mt][0] = *reinterpret_cast<const floatx4*>(base);
+          b.x[mt][1] = *reinterpret_cast<const floatx4*>(base + 16);
+        } else if constexpr (XL == 1) {
+          // kv and the lane's k are multiples of 4: a vector is all in or all out
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+            b.x[mt][h] = *reinterpret_cast<const floatx4*>(((okm >> (4 * h)) & 1u) ? base + 16 * h : X);
+        } else {
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int sidx = 0; sidx < 4; ++sidx)
+              b.x[mt][h][sidx] = *(((okm >> (4 * h + sidx)) & 1u) ? base + 16 * h + sidx : X);
+        }
       }
     } else {
       const float* base = ubase + (live ? loff[0] : 0);
@@ -447,7 +494,8 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
       for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int sidx = 0; sidx < 4; ++sidx) {
-          const float* p = base + (i64)(16 * h + sidx) * a.I;
+          const float* p = XL == 0 ? base + (i64)(16 * h + sidx) * a.I
+                                   : (((okm >> (4 * h + sidx)) & 1u) ? base + (i64)(16 * h + sidx) * a.I : X);
           if constexpr (MT == 1) {
             b.x[0][h][sidx] = p[0];
           } else if constexpr (MT == 2) {
@@ -468,6 +516,16 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
     if (t >= T) return;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
+      floatx4 xm[MT];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        xm[mt] = b.x[mt][h];
+        if constexpr (XL > 0) {
+#pragma unroll
+          for (int sidx = 0; sidx < 4; ++sidx)
+            if (!((b.okm >> (4 * h + sidx)) & 1u)) xm[mt][sidx] = 0.f;
+        }
+      }
       floatx4 bv[NT];
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
@@ -479,7 +537,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
           for (int mt = 0; mt < MT; ++mt)
-            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(b.x[mt][h][sidx], bv[nt][sidx], acc[mt][nt], 0, 0, 0);
+            acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(xm[mt][sidx], bv[nt][sidx], acc[mt][nt], 0, 0, 0);
     }
   };
 
@@ -487,6 +545,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
   load(0, b0);
   load(1, b1);
   load(2, b2);
+  if constexpr (XL > 0) window_zero_tail(fw, kv);
   stage_window_store<RT, PITCH, NTH, KWMAX>(sF, fw);
   __syncthreads();  // LDS window staged
   for (int t = 0; t < T; t += 4) {
@@ -518,7 +577,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
 //     result has the accuracy of an FP32 MTTKRP of the fp32 tensor.  The BF16 ALS
 //     runs its last mode, whose <M, A> gives the fit, this way.
 // ---------------------------------------------------------------------------
-template <typename XIN, int NT, int WAVES, int MT, int KS, int KWMAX, bool WV>
+template <typename XIN, int NT, int WAVES, int MT, int KS, int KWMAX, bool WV, int XL>
 __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __restrict__ X, KernArgs a,
                                                                float* __restrict__ partial, AlsJob job, int has_job) {
   constexpr bool X3 = std::is_same<XIN, float>::value;
@@ -561,7 +620,11 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __rest
   // balanced split of the slow range: the first slow_rem splits take one more
   const i64 s_lo = uniform_i64((i64)ss * a.slow_q + (ss < a.slow_rem ? ss : a.slow_rem));
   const i64 s_hi = s_lo + a.slow_q + (ss < a.slow_rem ? 1 : 0);
-  const int nkb = a.kw / KB;
+  // rows of this window that exist (XL == 1: a ragged fastest mode, whose loads
+  // are masked past kv; kv is a multiple of the 8- (bf16) or 4-element (fp32)
+  // load vector)
+  const int kv = XL == 0 ? a.kw : (int)min((i64)a.kw, a.fast_size - (i64)w * a.kw);
+  const int nkb = XL == 0 ? a.kw / KB : (kv + KB - 1) / KB;
   // the workgroup's blocks (slow index, then block within it) in WAVES contiguous,
   // balanced ranges: with 9 slow indices of 4 blocks, 9 blocks per wave rather
   // than 12 for one wave and 8 for the others
@@ -571,7 +634,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __rest
   const i64 row_base = (i64)xt * ROWS;
 
   FastWindow<RT, NTH, KWMAX> fw;  // window loads before the X blocks
-  stage_window_load<WV>(fw, a, w);
+  stage_window_load<WV>(fw, a, w, kv);
 
   i64 rowc[MT];
 #pragma unroll
@@ -593,6 +656,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __rest
     XV x[MT][KS];
     float pre[NT];
     int kb;
+    uint32_t okm;  // XL == 1: bit 2ks + v set where vector v of k-step ks lies below kv
   };
 
   // per-lane element offsets, fixed for the whole loop; the per-block part of
@@ -618,17 +682,29 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __rest
     for (int nt = 0; nt < NT; ++nt) b.pre[nt] = prow[poff[nt]];
     const bool live = t < T;  // past the end: one cache line at X (see the FP32 kernel)
     const XIN* ubase = X + (live ? cur.offset(a) : 0);
+    // XL == 1: vectors at k >= kv are loaded from X instead and zeroed in compute()
+    uint32_t okm = 0xffu;
+    if constexpr (XL > 0) {
+      okm = 0;
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+          okm |= (live && b.kb * KB + 32 * ks + 8 * g + 4 * v < kv) ? 1u << (2 * ks + v) : 0u;
+    }
+    b.okm = okm;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
       const XIN* base = ubase + (live ? loff[mt] : 0);
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
         if constexpr (X3) {
-          const floatx4 v0 = *reinterpret_cast<const floatx4*>(base + 32 * ks);
-          const floatx4 v1 = *reinterpret_cast<const floatx4*>(base + 32 * ks + 4);
+          const floatx4 v0 = *reinterpret_cast<const floatx4*>(((okm >> (2 * ks)) & 1u) ? base + 32 * ks : X);
+          const floatx4 v1 =
+              *reinterpret_cast<const floatx4*>(((okm >> (2 * ks + 1)) & 1u) ? base + 32 * ks + 4 : X);
           b.x[mt][ks] = floatx8{v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
         } else {
-          b.x[mt][ks] = *reinterpret_cast<const bf16x8*>(base + 32 * ks);
+          b.x[mt][ks] = *reinterpret_cast<const bf16x8*>(((okm >> (2 * ks)) & 1u) ? base + 32 * ks : X);
         }
       }
     }
@@ -642,7 +718,15 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __rest
       if constexpr (X3) {
         bf16x8 xh[MT], xl[MT], xl2[MT];
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) split_bf16x3(b.x[mt][ks], xh[mt], xl[mt], xl2[mt]);
+        for (int mt = 0; mt < MT; ++mt) {
+          floatx8 xv = b.x[mt][ks];
+          if constexpr (XL > 0) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              if (!((b.okm >> (2 * ks + e / 4)) & 1u)) xv[e] = 0.f;
+          }
+          split_bf16x3(xv, xh[mt], xl[mt], xl2[mt]);
+        }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
           const int off = (16 * nt + c) * PITCH + b.kb * KB + 32 * ks + 8 * g;
@@ -669,8 +753,12 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __rest
           const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&sLo[off]);
 #pragma unroll
           for (int mt = 0; mt < MT; ++mt) {
-            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b.x[mt][ks], bh, Tacc[mt][nt], 0, 0, 0);
-            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b.x[mt][ks], bl, Tacc[mt][nt], 0, 0, 0);
+            bf16x8 xv = b.x[mt][ks];
+            if constexpr (XL > 0) {
+              if (!((b.okm >> (2 * ks)) & 1u)) xv = bf16x8{};
+            }
+            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xv, bh, Tacc[mt][nt], 0, 0, 0);
+            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xv, bl, Tacc[mt][nt], 0, 0, 0);
           }
         }
       }
@@ -683,6 +771,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __rest
   load(0, b0);
   load(1, b1);
   load(2, b2);
+  if constexpr (XL > 0) window_zero_tail(fw, kv);
   if constexpr (X3) stage_window_store_bf16x3<RT, PITCH, NTH, KWMAX>(sHi, sLo, sLo2, fw);
   else stage_window_store_bf16x2<RT, PITCH, NTH, KWMAX>(sHi, sLo, fw);
   __syncthreads();  // LDS window staged
@@ -1152,17 +1241,31 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
   const bool caseB = pl.Q == 1;
   const int KB = (bf16 && !x3) ? ((pl.fast_size % 64 == 0) ? 64 : 32) : 32;
   constexpr int KWMAX = 128;  // LDS window bound (fp32: 64 x 132 floats)
-  // LDS window of the fastest mode: the largest multiple of KB that divides it
+  // LDS window of the fastest mode: the largest multiple of KB that divides it.
+  // A fastest mode no such window divides ("ragged", e.g. 100 or 250) takes
+  // windows of a multiple of 32 rows covering it as evenly as possible, the
+  // last one partly filled, with the loads past its end masked (XL > 0).
   int kw = 0;
   for (i64 cand = std::min<i64>(KWMAX, pl.fast_size) / KB * KB; cand >= KB; cand -= KB)
     if (pl.fast_size % cand == 0) {
       kw = (int)cand;
       break;
     }
+  int XL = 0;
+  if (kw == 0 && pl.fast_size > 0) {
+    const i64 nw = (pl.fast_size + KWMAX - 1) / KWMAX;
+    kw = (int)(((pl.fast_size + nw - 1) / nw + 31) / 32 * 32);
+    // masked vectors need the fastest mode (and so every row start) to be a
+    // multiple of the load vector: 4 fp32 or 8 bf16 elements; otherwise fp32
+    // loads element by element (XL 2) and bf16 takes the generic kernel
+    const int vecw = (bf16 && !x3) ? 8 : 4;
+    XL = (caseB || pl.fast_size % vecw == 0) ? 1 : 2;
+  }
   // the MFMA kernels read X with 8- and 16-byte vector loads: X must be 16-byte
-  // aligned (any view at another offset takes the generic kernel)
-  const bool mfma_ok =
-      pl.n_oth >= 1 && pl.J > 0 && kw > 0 && !(bf16 && caseB) && I > 0 && (uintptr_t)Xuse % 16 == 0;
+  // aligned (any view at another offset takes the generic kernel), 4-byte
+  // aligned for the element-wise form
+  const bool mfma_ok = pl.n_oth >= 1 && pl.J > 0 && kw > 0 && !(bf16 && caseB) && I > 0 &&
+                       (uintptr_t)Xuse % (XL == 2 ? 4 : 16) == 0 && !(bf16 && XL == 2);
 
   float* partial = nullptr;
   int splits = 1;
@@ -1178,45 +1281,52 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     const i64 rtiles = (i64)cdiv(I, rows_per_wg);
     const i64 tiles = rtiles * rchunks;
     a.kw = kw;
-    a.nwin = (int)(pl.fast_size / kw);
+    a.nwin = (int)((pl.fast_size + kw - 1) / kw);
     a.n_slow = pl.J / pl.fast_size;
     a.qs = caseB ? 1 : pl.Q / pl.fast_size;
     a.rtiles = (int)rtiles;
     a.win_vec = (R % 4 == 0) && ((uintptr_t)fastf % 16 == 0);
     // the kernel instance for this shape: calls f(kernel) with its function pointer
     auto with_kernel = [&](auto&& f) {
-      auto pick = [&](auto wv) {
+      auto pick = [&](auto wv, auto xl) {
         constexpr bool WV = decltype(wv)::value;
+        constexpr int L = decltype(xl)::value;  // XL; 2 only for fp32 case A
+        constexpr int LB = L == 2 ? 1 : L;      // instances that have no element-wise form
         if (!bf16) {
           if (!caseB) {
-            if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, false, KWMAX, WV>);
-            if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 2, false, KWMAX, WV>);
-            return f(mttkrp_f32_mfma<4, WAVES, 2, false, KWMAX, WV>);
+            if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, false, KWMAX, WV, L>);
+            if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 2, false, KWMAX, WV, L>);
+            return f(mttkrp_f32_mfma<4, WAVES, 2, false, KWMAX, WV, L>);
           }
           if (MT == 2) {
-            if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, true, KWMAX, WV>);
-            if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 2, true, KWMAX, WV>);
-            return f(mttkrp_f32_mfma<4, WAVES, 2, true, KWMAX, WV>);
+            if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, true, KWMAX, WV, LB>);
+            if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 2, true, KWMAX, WV, LB>);
+            return f(mttkrp_f32_mfma<4, WAVES, 2, true, KWMAX, WV, LB>);
           }
-          if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 1, true, KWMAX, WV>);
-          if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 1, true, KWMAX, WV>);
-          return f(mttkrp_f32_mfma<4, WAVES, 1, true, KWMAX, WV>);
+          if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 1, true, KWMAX, WV, LB>);
+          if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 1, true, KWMAX, WV, LB>);
+          return f(mttkrp_f32_mfma<4, WAVES, 1, true, KWMAX, WV, LB>);
         }
         if (x3) {
-          if (NT == 1) return f(mttkrp_bf16_mfma<float, 1, WAVES_BF16, 2, 1, KWMAX, WV>);
-          if (NT == 2) return f(mttkrp_bf16_mfma<float, 2, WAVES_BF16, 2, 1, KWMAX, WV>);
-          return f(mttkrp_bf16_mfma<float, 4, WAVES_BF16, 2, 1, KWMAX, WV>);
+          if (NT == 1) return f(mttkrp_bf16_mfma<float, 1, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
+          if (NT == 2) return f(mttkrp_bf16_mfma<float, 2, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
+          return f(mttkrp_bf16_mfma<float, 4, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
         }
-        if (KB == 64) {
-          if (NT == 1) return f(mttkrp_bf16_mfma<uint16_t, 1, WAVES_BF16, 2, 2, KWMAX, WV>);
-          if (NT == 2) return f(mttkrp_bf16_mfma<uint16_t, 2, WAVES_BF16, 2, 2, KWMAX, WV>);
-          return f(mttkrp_bf16_mfma<uint16_t, 4, WAVES_BF16, 2, 2, KWMAX, WV>);
+        if (KB == 64) {  // only a fastest mode divisible by 64 (never ragged)
+          if (NT == 1) return f(mttkrp_bf16_mfma<uint16_t, 1, WAVES_BF16, 2, 2, KWMAX, WV, 0>);
+          if (NT == 2) return f(mttkrp_bf16_mfma<uint16_t, 2, WAVES_BF16, 2, 2, KWMAX, WV, 0>);
+          return f(mttkrp_bf16_mfma<uint16_t, 4, WAVES_BF16, 2, 2, KWMAX, WV, 0>);
         }
-        if (NT == 1) return f(mttkrp_bf16_mfma<uint16_t, 1, WAVES_BF16, 2, 1, KWMAX, WV>);
-        if (NT == 2) return f(mttkrp_bf16_mfma<uint16_t, 2, WAVES_BF16, 2, 1, KWMAX, WV>);
-        return f(mttkrp_bf16_mfma<uint16_t, 4, WAVES_BF16, 2, 1, KWMAX, WV>);
+        if (NT == 1) return f(mttkrp_bf16_mfma<uint16_t, 1, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
+        if (NT == 2) return f(mttkrp_bf16_mfma<uint16_t, 2, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
+        return f(mttkrp_bf16_mfma<uint16_t, 4, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
       };
-      return a.win_vec ? pick(std::true_type{}) : pick(std::false_type{});
+      auto pick_xl = [&](auto wv) {
+        if (XL == 0) return pick(wv, std::integral_constant<int, 0>{});
+        if (XL == 1) return pick(wv, std::integral_constant<int, 1>{});
+        return pick(wv, std::integral_constant<int, 2>{});
+      };
+      return a.win_vec ? pick_xl(std::true_type{}) : pick_xl(std::false_type{});
     };
     // workgroups per CU doing MTTKRP work: 2 four-wave FP32 workgroups (the sweet
     // spot swept at 256^3 and 512x512x128), one eight-wave BF16 workgroup, fewer if

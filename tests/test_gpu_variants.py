@@ -63,8 +63,9 @@ def test_mttkrp_variants_match_standard_and_oracle(shape, rank):
             assert got.shape == (shape[mode], rank), name
             assert torch.equal(got, std), name
             assert rel(got.cpu().numpy(), ref) <= 1e-5, name
-        # the oracle's restatement of the blocked-parallel summation order agrees too
-        assert rel(oracle.mttkrp_blocked_parallel(X64, F64, mode, tile=5), ref) <= 1e-12
+        # the oracle's f32 restatement of the blocked-parallel summation order
+        # agrees to f32 rounding
+        assert rel(oracle.mttkrp_blocked_parallel(X, F, mode, tile=5), ref) <= 1e-6
     for fn in (kernels.mttkrp_blocked, kernels.mttkrp_blocked_parallel):
         with pytest.raises(ValueError, match="Tile size must be greater than 0"):
             fn(Xd, Fd, 0, 0)
