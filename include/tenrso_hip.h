@@ -117,6 +117,20 @@ int tenrso_convert_f32_to_bf16(tenrso_ctx* ctx, const float* src, int64_t n, uin
 int tenrso_mttkrp_profile(tenrso_ctx* ctx, int precision, const void* X, int ndim, const int64_t* shape,
                           const float* const* factors, int64_t rank, int mode, int reps, double* ms);
 
+/* No-allocation MTTKRP: the contract of tenrso_mttkrp_f32, but its device
+ * workspace (split partial sums; Khatri-Rao rows of the slow modes for tensors
+ * of order >= 4) lives in a file-scope device array of 16 MiB per GPU, held
+ * until the process ends, instead of the context's buffers: the call makes no
+ * hipMalloc, hipFree or other allocating call.  No counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  The context must have been created
+ * beforehand (tenrso_ctx_create sets the workspace up on its device).  A shape
+ * whose workspace exceeds 16 MiB even with the fewest split partials is
+ * TENRSO_ERR_UNSUPPORTED.  All calls on one GPU share the array: calls that
+ * overlap on one GPU race, so callers serialise them. */
+int tenrso_mttkrp_f32_noalloc(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                              const float* const* factors, int64_t rank, int mode, float* out);
+
 /* mttkrp_all_modes (tenrso-kernels/src/utils.rs:282-311): outs[k] for every mode. */
 int tenrso_mttkrp_all_modes_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
                                 const float* const* factors, int64_t rank, float* const* outs);

@@ -62,6 +62,7 @@ SIGNATURES = {
     "tenrso_mttkrp_bf16": ([_vp, _vp, _i, _vp, _vp, _i64, _i, _vp], _i),
     "tenrso_convert_f32_to_bf16": ([_vp, _vp, _i64, _vp], _i),
     "tenrso_mttkrp_profile": ([_vp, _i, _vp, _i, _vp, _vp, _i64, _i, _i, C.POINTER(_d)], _i),
+    "tenrso_mttkrp_f32_noalloc": ([_vp, _vp, _i, _vp, _vp, _i64, _i, _vp], _i),
     "tenrso_mttkrp_all_modes_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _vp], _i),
     "tenrso_khatri_rao_f32": ([_vp, _vp, _i64, _vp, _i64, _i64, _vp], _i),
     "tenrso_khatri_rao_f32_host": ([_vp, _vp, _i64, _vp, _i64, _i64, _vp], _i),

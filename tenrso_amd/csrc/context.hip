@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "common.hpp"
+#include "mttkrp.hpp"
 
 namespace tenrso {
 
@@ -78,6 +79,14 @@ int tenrso_ctx_create(int device, int borrow_stream, void* stream, tenrso_ctx** 
       set_error("device %d is %s; this library is built for gfx950 (MI355X)", device, prop.gcnArchName);
       delete c;
       return TENRSO_ERR_UNSUPPORTED;
+    }
+  }
+  {
+    // the no-allocation entry point's workspace address on this device
+    const int rc = noalloc_scratch_init(device);
+    if (rc != TENRSO_OK) {
+      delete c;
+      return rc;
     }
   }
   if (borrow_stream) {

@@ -42,6 +42,9 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
 // Size of the file-scope workspace of the no-allocation MTTKRP entry point
 // (noalloc.hip): 16 MiB per GPU.
 constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+// Resolves that workspace's address on `device` (which must be current); called
+// once per device by tenrso_ctx_create.
+int noalloc_scratch_init(int device);
 
 // Validation shared by the MTTKRP entry points (mode range, sizes, at least 2
 // modes), with the reference's messages.
