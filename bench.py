@@ -188,6 +188,21 @@ def extra_configs(with_cpu):
     out["dense_250_r64_ragged"] = cr
     del Xd
 
+    # out-of-core MTTKRP: a 512^3 fp32 host tensor (512 MiB, pinned) streamed in
+    # mode-0 slabs through a 256 MiB device budget, rank 32
+    g = torch.Generator().manual_seed(11)
+    Xh = torch.randn((512, 512, 512), generator=g, dtype=torch.float32).pin_memory()
+    Fo = [torch.randn((512, 32), generator=g, dtype=torch.float32).numpy() for _ in range(3)]
+    kernels.mttkrp_streaming(Xh, Fo, 0, 256 << 20)
+    oo = {}
+    for mode in range(3):
+        _, st = kernels.mttkrp_streaming(Xh, Fo, mode, 256 << 20, return_stats=True)
+        oo[str(mode)] = {"ms": st["seconds"] * 1e3, "h2d_GBs": st["h2d_GBs"], "slabs": st["slabs"],
+                         "slab_rows": st["slab_rows"]}
+    out["ooc_512cube_r32_budget256MiB"] = {"shape": [512, 512, 512], "rank": 32, "pinned_input": True,
+                                           "per_mode": oo}
+    del Xh
+
     # 96^4 rank 32: MTTKRP per mode, CP-loss gradient, gradient-descent steps
     X, _, _, init = synth.dense4_96_r32()
     Xd = torch.from_numpy(X).to(dev)
@@ -232,8 +247,13 @@ def extra_configs(with_cpu):
             sp.mttkrp(Fd, mode)
             ts.append(sp.ctx.last_kernel_ms())
         t = float(np.median(ts))
-        alg = nnz * (3 * 4 + 4) + sum(shape[k] * R * 4 for k in range(3) if k != mode) + shape[mode] * R * 4
-        modes[str(mode)] = {"ms": t, "GBs": alg / (t * 1e-3) / 1e9, "algorithmic_bytes": alg}
+        fac = sum(shape[k] * R * 4 for k in range(3) if k != mode) + shape[mode] * R * 4
+        alg = nnz * (3 * 4 + 4) + fac  # the sorted COO form (row, two indices, value per nonzero)
+        nfib = sp.fibres(mode)
+        csf = nnz * 8 + nfib * 4 * 3 + fac  # what the fibre kernel reads: leaves, fibre starts and prefixes
+        modes[str(mode)] = {"ms": t, "GBs": alg / (t * 1e-3) / 1e9, "algorithmic_bytes": alg,
+                            "fibres": nfib, "csf_bytes": csf, "csf_GBs": csf / (t * 1e-3) / 1e9,
+                            "csf_frac_of_hbm": csf / (t * 1e-3) / 1e9 / PEAK_HBM_GBS}
     sp.ctx.set_timing(False)
     init = [rng.standard_normal((n, R)).astype(np.float32) for n in shape]
     sp.cp_als_with_init(init, 2, 0.0)

@@ -131,6 +131,22 @@ int tenrso_mttkrp_profile(tenrso_ctx* ctx, int precision, const void* X, int ndi
 int tenrso_mttkrp_f32_noalloc(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
                               const float* const* factors, int64_t rank, int mode, float* out);
 
+/* Out-of-core MTTKRP: the streaming MTTKRP that tenrso-ooc's example describes
+ * (tenrso-ooc/examples/mttkrp_streaming.rs:66-110: chunk the tensor, a partial
+ * MTTKRP per chunk, accumulate; the example itself calls the in-memory mttkrp).
+ * X is a HOST tensor (pageable, or pinned with hipHostMalloc / torch pin_memory:
+ * then copied without staging), factors and out are host arrays.  The tensor
+ * streams in mode-0 slabs through two device buffers sized so that this call's
+ * device memory stays within device_budget_bytes (including the split partial
+ * sums of the per-slab MFMA MTTKRP), copies overlapped with the previous slab's
+ * kernels; mode-n > 0 partials are summed in FP64 in slab order.  stats (may be
+ * NULL, 8 doubles): seconds, tensor bytes moved host->device per second (GB/s),
+ * slab rows, slabs, input pinned (1/0), host staging seconds.  A budget below
+ * two one-row slabs plus the factors and partials is TENRSO_ERR_INVALID_ARGUMENT. */
+int tenrso_mttkrp_f32_streaming(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                                const float* const* factors, int64_t rank, int mode, float* out,
+                                int64_t device_budget_bytes, double* stats);
+
 /* mttkrp_all_modes (tenrso-kernels/src/utils.rs:282-311): outs[k] for every mode. */
 int tenrso_mttkrp_all_modes_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
                                 const float* const* factors, int64_t rank, float* const* outs);
@@ -322,6 +338,12 @@ int tenrso_sp_csf(tenrso_ctx* ctx, const tenrso_sptensor* t, int mode, int64_t* 
  * A zero in block_shape is TENRSO_ERR_INVALID_ARGUMENT (hicoo.rs:139-143). */
 int tenrso_sp_hicoo(tenrso_ctx* ctx, const tenrso_sptensor* t, const int64_t* block_shape, int64_t* num_blocks,
                     int64_t* block_coords, int64_t* block_ptrs, int32_t* local, float* values);
+/* Number of compressed fibres of a mode: the nodes one level above the leaves of
+ * the CSF of mode order [mode, others ascending] (csf.rs:137-264), i.e. the
+ * distinct prefixes (i_mode, i_o1, .., i_o(N-2)) of the nonzeros.  The sparse
+ * MTTKRP of 2- to 4-way tensors reads these fibres (start and prefix, 4 (N)
+ * bytes each) and the leaves (index and value, 8 bytes per nonzero). */
+int tenrso_sp_fibres(const tenrso_sptensor* t, int mode, int64_t* nfib);
 /* Sparse MTTKRP (no reference counterpart; contract of the dense mttkrp on the
  * tensor the COO represents).  factors / out are device pointers. */
 int tenrso_sp_mttkrp_f32(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* factors, int64_t rank,

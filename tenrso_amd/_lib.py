@@ -64,6 +64,7 @@ SIGNATURES = {
     "tenrso_mttkrp_profile": ([_vp, _i, _vp, _i, _vp, _vp, _i64, _i, _i, C.POINTER(_d)], _i),
     "tenrso_mttkrp_f32_noalloc": ([_vp, _vp, _i, _vp, _vp, _i64, _i, _vp], _i),
     "tenrso_mttkrp_all_modes_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _vp], _i),
+    "tenrso_mttkrp_f32_streaming": ([_vp, _vp, _i, _vp, _vp, _i64, _i, _vp, _i64, _vp], _i),
     "tenrso_khatri_rao_f32": ([_vp, _vp, _i64, _vp, _i64, _i64, _vp], _i),
     "tenrso_khatri_rao_f32_host": ([_vp, _vp, _i64, _vp, _i64, _i64, _vp], _i),
     "tenrso_cp_reconstruct_f32": ([_vp, _i, _vp, _vp, _i64, _vp, _vp], _i),
@@ -89,6 +90,7 @@ SIGNATURES = {
     "tenrso_sp_mode_arrays": ([_vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
     "tenrso_sp_csf": ([_vp, _vp, _i, _vp, _vp, _vp, _vp], _i),
     "tenrso_sp_hicoo": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    "tenrso_sp_fibres": ([_vp, _i, _vp], _i),
     "tenrso_sp_mttkrp_f32": ([_vp, _vp, _vp, _i64, _i, _vp], _i),
     "tenrso_sp_cp_als_f32": ([_vp, _vp, _vp, C.POINTER(CpOptions), C.POINTER(CpResult), _vp], _i),
 }

@@ -486,6 +486,16 @@ int tenrso_sp_hicoo(tenrso_ctx* ctx, const tenrso_sptensor* t, const int64_t* bl
   return TENRSO_OK;
 }
 
+int tenrso_sp_fibres(const tenrso_sptensor* t, int mode, int64_t* nfib) {
+  clear_error();
+  if (!t || !nfib || mode < 0 || mode >= t->nd) {
+    set_error("null tensor or output, or mode out of range");
+    return TENRSO_ERR_INVALID_ARGUMENT;
+  }
+  *nfib = t->modes[mode].nfib;
+  return TENRSO_OK;
+}
+
 int tenrso_sp_mttkrp_f32(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* factors, int64_t rank,
                          int mode, float* out) {
   clear_error();

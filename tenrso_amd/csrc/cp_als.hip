@@ -322,13 +322,23 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
   // the shape allows it (prepare_bf16_als); otherwise norm, copy and transposed
   // copy are three passes
   bool prepared = false;
+  // A last mode that is no multiple of 8 (bf16 copy) or a mode N-2 that is no
+  // multiple of 4 (the fp32 last-to-front copy, whose last mode it is) would
+  // leave rows off the MFMA kernels' load vectors: those copies get the mode
+  // padded with zeros (ragged shapes such as 250^3), so every mode still runs
+  // on the matrix cores.
+  const i64 Kx = shape[nd - 1], Bt = nd >= 2 ? shape[nd - 2] : 1;
+  const i64 Kxp = (Kx + 7) / 8 * 8, Btp = (Bt + 3) / 4 * 4;
+  const bool padded = bf16 && nd >= 2 && total > 0 && (Kxp != Kx || Btp != Bt);
   auto bf16_buffers = [&](void** pb16, void** pt) -> int {
-    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16, (size_t)total * 2, pb16));
-    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * sizeof(float), pt));
+    const size_t nb = padded ? (size_t)(total / Kx * Kxp) : (size_t)total;
+    const size_t nt = padded ? (size_t)(total / Bt * Btp) : (size_t)total;
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16, nb * 2, pb16));
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, nt * sizeof(float), pt));
     return TENRSO_OK;
   };
   pb.norm_min = [&](double* tmp, double* scal) -> int {
-    if (bf16) {
+    if (bf16 && !padded) {
       void *pb16, *pt;
       TENRSO_TRY(bf16_buffers(&pb16, &pt));
       int nblk = 0;
@@ -345,15 +355,21 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
     if (!bf16 || prepared) return TENRSO_OK;
     void *pb16, *pt;
     TENRSO_TRY(bf16_buffers(&pb16, &pt));
-    TENRSO_TRY(convert_bf16(ctx, X, nd, shape, false, (uint16_t*)pb16));
-    TENRSO_TRY(transpose_last_to_front_f32(ctx, X, nd, shape, (float*)pt));
+    if (padded) {
+      TENRSO_TRY(convert_bf16_padded(ctx, X, nd, shape, Kxp, (uint16_t*)pb16));
+      TENRSO_TRY(transpose_last_to_front_f32_padded(ctx, X, nd, shape, Btp, (float*)pt));
+    } else {
+      TENRSO_TRY(convert_bf16(ctx, X, nd, shape, false, (uint16_t*)pb16));
+      TENRSO_TRY(transpose_last_to_front_f32(ctx, X, nd, shape, (float*)pt));
+    }
     Xk = pb16;
     Xt = pt;
     return TENRSO_OK;
   };
   pb.mttkrp = [&](int mode, const float* const* Fc, const AlsJob* job, MttkrpResult* mr) {
     return mttkrp_dense(ctx, bf16 ? TENRSO_PREC_BF16 : TENRSO_PREC_FP32, Xk, Xt, nd, shape, Fc, R, mode,
-                        MttkrpOut::PARTIAL, nullptr, mr, job, /*xt_f32=*/true);
+                        MttkrpOut::PARTIAL, nullptr, mr, job, /*xt_f32=*/true, nullptr, padded ? Kxp : 0,
+                        padded ? Btp : 0);
   };
   return cp_als_run(ctx, pb, F, o, res, hist);
 }

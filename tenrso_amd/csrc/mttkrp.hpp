@@ -35,9 +35,12 @@ struct MttkrpScratch {
 // the MFMA kernel contracts with a three-way bf16 split at FP32 accuracy.
 // job (optional): the ALS linear algebra of this mode; it runs as workgroup 0 of the
 // MTTKRP launch (or, where it cannot, as its own kernel just before it).
+// x_pad / xt_pad (BF16 only, 0 = none): the memory extent of the last mode of X
+// and of X_t when those copies have it padded with zeros.
 int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int nd, const i64* shape,
                  const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res,
-                 const AlsJob* job = nullptr, bool xt_f32 = false, const MttkrpScratch* scratch = nullptr);
+                 const AlsJob* job = nullptr, bool xt_f32 = false, const MttkrpScratch* scratch = nullptr,
+                 i64 x_pad = 0, i64 xt_pad = 0);
 
 // Size of the file-scope workspace of the no-allocation MTTKRP entry point
 // (noalloc.hip): 16 MiB per GPU.
@@ -69,6 +72,12 @@ int prepare_bf16_als(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, 
 // [.., I_{N-1}] -> [I_{N-1}, ..] (last mode moved to the front), plain copies.
 int transpose_last_to_front_u16(tenrso_ctx* ctx, const uint16_t* src, int nd, const i64* shape, uint16_t* dst);
 int transpose_last_to_front_f32(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, float* dst);
+// Padded layouts for the BF16 ALS of tensors with ragged last modes: the bf16
+// copy with its last mode padded to Kp, and the fp32 last-to-front copy with
+// its new last mode (shape[nd-2]) padded to Bp; pad entries are zero.
+int convert_bf16_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, i64 Kp, uint16_t* dst);
+int transpose_last_to_front_f32_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, i64 Bp,
+                                       float* dst);
 
 // Forward-order Khatri-Rao of `n` factors (rows = prod sizes) into dst (fp32).
 int khatri_rao_many(tenrso_ctx* ctx, int n, const float* const* F, const i64* sizes, i64 R, float* dst);

@@ -46,7 +46,9 @@ struct KernArgs {
   i64 I, Q;           // X viewed as [P, I, Q]
   i64 R;              // factor row stride (rank)
   i64 fast_size;      // size of the fastest other mode
-  i64 qs;             // Q / fast_size: slow indices per p (case A)
+  i64 fast_stride;    // its extent in memory: fast_size, or more in a copy whose last
+                      // mode is padded (Q is then the padded extent too)
+  i64 qs;             // Q / fast_stride: slow indices per p (case A)
   const float* pre;   // PRE (rows = J / fast_size, stride R) or null (= 1)
   const float* fast;  // FAST factor (fast_size x R)
   int kw;             // window of the fastest mode held in LDS (multiple of KB, divides fast_size)
@@ -250,8 +252,8 @@ struct BlockCursor {
       // 32-bit division whenever the slow range fits (it nearly always does)
       const i64 p = (a.n_slow >> 32) ? slow / a.qs : (i64)((uint32_t)slow / (uint32_t)a.qs);
       sq = slow - p * a.qs;
-      base = p * a.I * a.Q + sq * a.fast_size + (i64)w * a.kw;
-      wrap = a.I * a.Q - a.qs * a.fast_size;  // from (p, qs) to (p + 1, 0)
+      base = p * a.I * a.Q + sq * a.fast_stride + (i64)w * a.kw;
+      wrap = a.I * a.Q - a.qs * a.fast_stride;  // from (p, qs) to (p + 1, 0)
     } else {
       sq = 0;
       wrap = 0;
@@ -267,7 +269,7 @@ struct BlockCursor {
     if (++kb == nkb) {
       kb = 0;
       slow += STRIDE;
-      base += (i64)STRIDE * a.fast_size;
+      base += (i64)STRIDE * a.fast_stride;
       if constexpr (!CASE_B) {
         sq += STRIDE;
         while (sq >= a.qs) {
@@ -727,39 +729,63 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __rest
           }
           split_bf16x3(xv, xh[mt], xl[mt], xl2[mt]);
         }
+        // Two rank tiles at a time, term by term (smallest first): consecutive
+        // MFMAs then update 2 * MT different accumulators, so none waits for the
+        // previous one's result (one accumulator taking all six terms in a row
+        // serialised the matrix pipe on its latency)
+        constexpr int NG = NT >= 2 ? 2 : 1;
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          const int off = (16 * nt + c) * PITCH + b.kb * KB + 32 * ks + 8 * g;
-          const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&sHi[off]);
-          const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&sLo[off]);
-          const bf16x8 bl2 = *reinterpret_cast<const bf16x8*>(&sLo2[off]);
+        for (int n0 = 0; n0 < NT; n0 += NG) {
+          bf16x8 bh[NG], bl[NG], bl2[NG];
 #pragma unroll
-          for (int mt = 0; mt < MT; ++mt) {
-            // smallest terms first
-            floatx4 d = Tacc[mt][nt];
-            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl2[mt], bh, d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[mt], bl, d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[mt], bl2, d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[mt], bh, d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[mt], bl, d, 0, 0, 0);
-            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[mt], bh, d, 0, 0, 0);
+          for (int u = 0; u < NG; ++u) {
+            const int off = (16 * (n0 + u) + c) * PITCH + b.kb * KB + 32 * ks + 8 * g;
+            bh[u] = *reinterpret_cast<const bf16x8*>(&sHi[off]);
+            bl[u] = *reinterpret_cast<const bf16x8*>(&sLo[off]);
+            bl2[u] = *reinterpret_cast<const bf16x8*>(&sLo2[off]);
           }
+#define TENRSO_X3_TERM(XA, XB)                                                                        \
+  _Pragma("unroll") for (int u = 0; u < NG; ++u) _Pragma("unroll") for (int mt = 0; mt < MT; ++mt)     \
+      Tacc[mt][n0 + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(XA[mt], XB[u], Tacc[mt][n0 + u], 0, 0, 0);
+          TENRSO_X3_TERM(xl2, bh)
+          TENRSO_X3_TERM(xl, bl)
+          TENRSO_X3_TERM(xh, bl2)
+          TENRSO_X3_TERM(xl, bh)
+          TENRSO_X3_TERM(xh, bl)
+          TENRSO_X3_TERM(xh, bh)
+#undef TENRSO_X3_TERM
         }
       } else {
+        bf16x8 xv[MT];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          const int off = (16 * nt + c) * PITCH + b.kb * KB + 32 * ks + 8 * g;
-          const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&sHi[off]);
-          const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&sLo[off]);
-#pragma unroll
-          for (int mt = 0; mt < MT; ++mt) {
-            bf16x8 xv = b.x[mt][ks];
-            if constexpr (XL > 0) {
-              if (!((b.okm >> (2 * ks)) & 1u)) xv = bf16x8{};
-            }
-            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xv, bh, Tacc[mt][nt], 0, 0, 0);
-            Tacc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xv, bl, Tacc[mt][nt], 0, 0, 0);
+        for (int mt = 0; mt < MT; ++mt) {
+          xv[mt] = b.x[mt][ks];
+          if constexpr (XL > 0) {
+            if (!((b.okm >> (2 * ks)) & 1u)) xv[mt] = bf16x8{};
           }
+        }
+        // two rank tiles at a time, the hi term of all four accumulators before
+        // their lo term (no MFMA waits for the one just issued)
+        constexpr int NG = NT >= 2 ? 2 : 1;
+#pragma unroll
+        for (int n0 = 0; n0 < NT; n0 += NG) {
+          bf16x8 bh[NG], bl[NG];
+#pragma unroll
+          for (int u = 0; u < NG; ++u) {
+            const int off = (16 * (n0 + u) + c) * PITCH + b.kb * KB + 32 * ks + 8 * g;
+            bh[u] = *reinterpret_cast<const bf16x8*>(&sHi[off]);
+            bl[u] = *reinterpret_cast<const bf16x8*>(&sLo[off]);
+          }
+#pragma unroll
+          for (int u = 0; u < NG; ++u)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+              Tacc[mt][n0 + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xv[mt], bh[u], Tacc[mt][n0 + u], 0, 0, 0);
+#pragma unroll
+          for (int u = 0; u < NG; ++u)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+              Tacc[mt][n0 + u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xv[mt], bl[u], Tacc[mt][n0 + u], 0, 0, 0);
         }
       }
     }
@@ -900,9 +926,12 @@ __device__ __forceinline__ uint16_t to_out(float v, uint16_t*) { return f32_to_b
 __device__ __forceinline__ uint16_t to_out(uint16_t v, uint16_t*) { return v; }
 __device__ __forceinline__ float to_out(float v, float*) { return v; }
 
+// With B < Bp the output rows are padded: input row r = a*B + b lands at
+// a*Bp + b of an output row of extent (rows / B) * Bp (the pad entries are
+// zero-filled beforehand); B = Bp leaves the layout plain.
 template <typename TIN, typename TOUT>
 __global__ __launch_bounds__(256) void transpose_kernel(const TIN* __restrict__ src, i64 rows, i64 K,
-                                                        TOUT* __restrict__ dst) {
+                                                        TOUT* __restrict__ dst, i64 B = 1, i64 Bp = 1) {
   __shared__ TIN tile[32][33];
   const i64 r0 = (i64)blockIdx.y * 32, k0 = (i64)blockIdx.x * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -913,7 +942,10 @@ __global__ __launch_bounds__(256) void transpose_kernel(const TIN* __restrict__ 
   __syncthreads();
   for (int yy = ty; yy < 32; yy += 8) {
     const i64 k = k0 + yy, r = r0 + tx;
-    if (k < K && r < rows) dst[k * rows + r] = to_out(tile[tx][yy], dst);
+    if (k < K && r < rows) {
+      const i64 o = B == Bp ? k * rows + r : k * (rows / B * Bp) + (r / B) * Bp + r % B;
+      dst[o] = to_out(tile[tx][yy], dst);
+    }
   }
 }
 
@@ -981,6 +1013,48 @@ int convert_bf16(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, bo
     hipLaunchKernelGGL((transpose_kernel<float, uint16_t>), dim3(cdiv(K, 32), cdiv(rows, 32)), dim3(256), 0,
                        ctx->stream, src, rows, K, dst);
   }
+  TENRSO_HIP_TRY(hipGetLastError());
+  return TENRSO_OK;
+}
+
+// [rows, K] fp32 -> [rows, Kp] bf16 (RNE), zero in the pad columns k >= K
+__global__ __launch_bounds__(256) void f32_to_bf16_pad_kernel(const float* __restrict__ src, i64 rows, i64 K,
+                                                              i64 Kp, uint16_t* __restrict__ dst) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= rows * Kp) return;
+  const i64 r = e / Kp, k = e - r * Kp;
+  dst[e] = k < K ? f32_to_bf16_rne(src[r * K + k]) : (uint16_t)0;
+}
+
+__global__ __launch_bounds__(256) void zero_u32_kernel(uint32_t* __restrict__ p, i64 n) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) p[e] = 0u;
+}
+
+int convert_bf16_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, i64 Kp, uint16_t* dst) {
+  i64 total = 1;
+  for (int k = 0; k < nd; ++k) total *= shape[k];
+  const i64 K = shape[nd - 1];
+  if (total == 0) return TENRSO_OK;
+  const i64 rows = total / K;
+  hipLaunchKernelGGL(f32_to_bf16_pad_kernel, dim3(cdiv(rows * Kp, 256)), dim3(256), 0, ctx->stream, src, rows, K, Kp,
+                     dst);
+  TENRSO_HIP_TRY(hipGetLastError());
+  return TENRSO_OK;
+}
+
+int transpose_last_to_front_f32_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, i64 Bp,
+                                       float* dst) {
+  i64 total = 1;
+  for (int k = 0; k < nd; ++k) total *= shape[k];
+  if (total == 0) return TENRSO_OK;
+  const i64 K = shape[nd - 1], rows = total / K, B = shape[nd - 2];
+  const i64 out_n = K * (rows / B) * Bp;
+  if (Bp != B)
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(cdiv(out_n, 256)), dim3(256), 0, ctx->stream,
+                       reinterpret_cast<uint32_t*>(dst), out_n);
+  hipLaunchKernelGGL((transpose_kernel<float, float>), dim3(cdiv(K, 32), cdiv(rows, 32)), dim3(256), 0, ctx->stream,
+                     src, rows, K, dst, B, Bp);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -1142,7 +1216,7 @@ Plan make_plan(int nd, const i64* shape, int mode) {
 
 int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int nd, const i64* shape,
                  const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res,
-                 const AlsJob* job, bool xt_f32, const MttkrpScratch* scratch) {
+                 const AlsJob* job, bool xt_f32, const MttkrpScratch* scratch, i64 x_pad, i64 xt_pad) {
   Plan pl = make_plan(nd, shape, mode);
   const bool bf16 = prec == TENRSO_PREC_BF16;
   const float* Fv[MAXD];
@@ -1230,15 +1304,23 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
   }
   const float* fastf = pl.n_oth ? Fv[pl.oth[pl.n_oth - 1]] : nullptr;
 
+  const bool caseB = pl.Q == 1;
+  // memory extent of the fastest other mode: the BF16 ALS copies of a tensor
+  // whose last mode is no multiple of the load vector have that mode padded
+  // (zeros), so every row starts on a vector boundary
+  const i64 pad = !bf16 ? 0 : (mode == nd - 1 && nd >= 2 ? xt_pad : x_pad);
+  const i64 fast_stride = (pad > 0 && !caseB) ? pad : pl.fast_size;
+  const i64 Qmem = (pad > 0 && !caseB) ? pl.Q / pl.fast_size * pad : pl.Q;
+
   KernArgs a{};
   a.I = I;
-  a.Q = pl.Q;
+  a.Q = Qmem;
   a.R = R;
   a.fast_size = pl.fast_size;
+  a.fast_stride = fast_stride;
   a.pre = pre;
   a.fast = fastf;
 
-  const bool caseB = pl.Q == 1;
   const int KB = (bf16 && !x3) ? ((pl.fast_size % 64 == 0) ? 64 : 32) : 32;
   constexpr int KWMAX = 128;  // LDS window bound (fp32: 64 x 132 floats)
   // LDS window of the fastest mode: the largest multiple of KB that divides it.
@@ -1259,13 +1341,17 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     // multiple of the load vector: 4 fp32 or 8 bf16 elements; otherwise fp32
     // loads element by element (XL 2) and bf16 takes the generic kernel
     const int vecw = (bf16 && !x3) ? 8 : 4;
-    XL = (caseB || pl.fast_size % vecw == 0) ? 1 : 2;
+    XL = (caseB || fast_stride % vecw == 0) ? 1 : 2;
   }
-  // the MFMA kernels read X with 8- and 16-byte vector loads: X must be 16-byte
-  // aligned (any view at another offset takes the generic kernel), 4-byte
-  // aligned for the element-wise form
+  // The MFMA kernels read X with 16-byte vectors (case A) or MT-wide vectors
+  // along i (case B).  An fp32 view that is only 4-byte aligned (a slice at an
+  // odd offset) loads element-wise in case A (XL 2 also takes windows that
+  // divide the mode: all its masks are set) and one row per lane in case B; the
+  // bf16 kernels need 16-byte alignment, else the generic kernel runs.
+  const bool aligned16 = (uintptr_t)Xuse % 16 == 0;
+  if (!bf16 && !caseB && !aligned16 && kw > 0) XL = 2;
   const bool mfma_ok = pl.n_oth >= 1 && pl.J > 0 && kw > 0 && !(bf16 && caseB) && I > 0 &&
-                       (uintptr_t)Xuse % (XL == 2 ? 4 : 16) == 0 && !(bf16 && XL == 2);
+                       (uintptr_t)Xuse % 4 == 0 && (bf16 ? aligned16 && XL != 2 : (caseB || XL == 2 || aligned16));
 
   float* partial = nullptr;
   int splits = 1;
@@ -1276,14 +1362,14 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     // ~22 us beside the MTTKRP against ~14 us alone).
     constexpr int WAVES = 4, WAVES_BF16 = 8;
     const int waves = bf16 ? WAVES_BF16 : WAVES;
-    const int MT = (caseB && (I % 2 != 0)) ? 1 : 2;
+    const int MT = (caseB && (I % 2 != 0 || (uintptr_t)Xuse % 8 != 0)) ? 1 : 2;
     const i64 rows_per_wg = 16 * MT;
     const i64 rtiles = (i64)cdiv(I, rows_per_wg);
     const i64 tiles = rtiles * rchunks;
     a.kw = kw;
     a.nwin = (int)((pl.fast_size + kw - 1) / kw);
     a.n_slow = pl.J / pl.fast_size;
-    a.qs = caseB ? 1 : pl.Q / pl.fast_size;
+    a.qs = caseB ? 1 : Qmem / fast_stride;
     a.rtiles = (int)rtiles;
     a.win_vec = (R % 4 == 0) && ((uintptr_t)fastf % 16 == 0);
     // the kernel instance for this shape: calls f(kernel) with its function pointer
@@ -1418,6 +1504,10 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
       TENRSO_TRY(launch_main(true));
     }
   } else {
+    if (pad > 0 && fast_stride != pl.fast_size) {
+      set_error("internal: a padded tensor copy reached the generic MTTKRP");
+      return TENRSO_ERR_UNSUPPORTED;
+    }
     if (job) TENRSO_TRY(als_job_launch(ctx, *job));
     GenericArgs ga{};
     ga.nd = nd;

@@ -23,6 +23,7 @@
 // until one tile remains.  No atomics: the result is deterministic.
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <vector>
@@ -106,6 +107,46 @@ __global__ __launch_bounds__(256) void bounds_kernel(KeyArgs a, int* __restrict_
   }
 }
 
+// ---- compressed fibres (see SpMode) ----------------------------------------
+// flag[e] = 1 where the prefix (row, oth[0 .. D-2]) of sorted nonzero e differs
+// from that of e - 1 (e = 0 always starts a fibre)
+__global__ __launch_bounds__(256) void fib_flag_kernel(const int32_t* __restrict__ row,
+                                                       const int32_t* __restrict__ oth, i64 nnz, int D,
+                                                       int32_t* __restrict__ flag) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= nnz) return;
+  int f = e == 0 || row[e] != row[e - 1];
+  for (int l = 0; l + 1 < D && !f; ++l) f = oth[(i64)l * nnz + e] != oth[(i64)l * nnz + e - 1];
+  flag[e] = f;
+}
+
+// pos = exclusive scan of flag: fibre f = pos[e] starts at e where flag[e]
+__global__ __launch_bounds__(256) void fib_scatter_kernel(const int32_t* __restrict__ flag,
+                                                          const int32_t* __restrict__ pos,
+                                                          const int32_t* __restrict__ row,
+                                                          const int32_t* __restrict__ oth, i64 nnz, int D, i64 nfib,
+                                                          int32_t* __restrict__ fstart, int32_t* __restrict__ fpre) {
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= nnz) return;
+  if (flag[e]) {
+    const i64 f = pos[e];
+    fstart[f] = (int32_t)e;
+    fpre[f] = row[e];
+    for (int l = 1; l < D; ++l) fpre[(i64)l * nfib + f] = oth[(i64)(l - 1) * nnz + e];
+  }
+  if (e == nnz - 1) fstart[nfib] = (int32_t)nnz;
+}
+
+// chunk_fib[c] = the fibre holding nonzero c * W (inclusive scan - 1)
+__global__ __launch_bounds__(256) void chunk_fib_kernel(const int32_t* __restrict__ flag,
+                                                        const int32_t* __restrict__ pos, i64 nchunks, int W,
+                                                        int32_t* __restrict__ cf) {
+  const i64 c = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (c >= nchunks) return;
+  const i64 e = c * W;
+  cf[c] = pos[e] + flag[e] - 1;
+}
+
 // Device temporaries of one build step, freed on every path out of it.
 struct DevTemps {
   std::vector<void*> ptrs;
@@ -119,6 +160,44 @@ struct DevTemps {
     for (void* p : ptrs) (void)hipFree(p);
   }
 };
+
+// The compressed fibres of one mode from its sorted nonzeros (flags, an
+// exclusive scan, a scatter of the fibre starts and prefixes, the chunk table).
+static int sp_build_fibres(tenrso_ctx* ctx, tenrso_sptensor* t, int mode) {
+  SpMode& md = t->modes[mode];
+  const i64 nnz = t->nnz;
+  const int D = t->nd - 1;
+  DevTemps tmps;
+  int32_t *flag, *pos;
+  TENRSO_TRY(tmps.alloc((void**)&flag, (size_t)nnz * 4));
+  TENRSO_TRY(tmps.alloc((void**)&pos, (size_t)nnz * 4));
+  hipLaunchKernelGGL(fib_flag_kernel, dim3(cdiv(nnz, 256)), dim3(256), 0, ctx->stream, md.row, md.oth, nnz, D, flag);
+  TENRSO_HIP_TRY(hipGetLastError());
+  size_t tmp_bytes = 0;
+  TENRSO_HIP_TRY(rocprim::exclusive_scan(nullptr, tmp_bytes, flag, pos, 0, (size_t)nnz, rocprim::plus<int32_t>(),
+                                         ctx->stream));
+  void* tmp = nullptr;
+  TENRSO_TRY(tmps.alloc(&tmp, std::max<size_t>(tmp_bytes, 16)));
+  TENRSO_HIP_TRY(rocprim::exclusive_scan(tmp, tmp_bytes, flag, pos, 0, (size_t)nnz, rocprim::plus<int32_t>(),
+                                         ctx->stream));
+  int32_t last[2] = {0, 0};
+  TENRSO_HIP_TRY(hipMemcpyAsync(&last[0], pos + nnz - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+  TENRSO_HIP_TRY(hipMemcpyAsync(&last[1], flag + nnz - 1, 4, hipMemcpyDeviceToHost, ctx->stream));
+  TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  md.nfib = (i64)last[0] + last[1];
+  md.nchunks = (nnz + SP_CSF_W - 1) / SP_CSF_W;
+  TENRSO_HIP_TRY(hipMalloc(&md.fstart, (size_t)(md.nfib + 1) * 4));
+  TENRSO_HIP_TRY(hipMalloc(&md.fpre, (size_t)std::max<i64>(1, md.nfib * D) * 4));
+  TENRSO_HIP_TRY(hipMalloc(&md.chunk_fib, (size_t)md.nchunks * 4));
+  hipLaunchKernelGGL(fib_scatter_kernel, dim3(cdiv(nnz, 256)), dim3(256), 0, ctx->stream, flag, pos, md.row, md.oth,
+                     nnz, D, md.nfib, md.fstart, md.fpre);
+  hipLaunchKernelGGL(chunk_fib_kernel, dim3(cdiv(md.nchunks, 256)), dim3(256), 0, ctx->stream, flag, pos, md.nchunks,
+                     SP_CSF_W, md.chunk_fib);
+  TENRSO_HIP_TRY(hipGetLastError());
+  // the temporaries are freed below: the kernels reading them must be done
+  TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return TENRSO_OK;
+}
 
 int sp_build_mode(tenrso_ctx* ctx, tenrso_sptensor* t, int mode) {
   SpMode& md = t->modes[mode];
@@ -166,6 +245,7 @@ int sp_build_mode(tenrso_ctx* ctx, tenrso_sptensor* t, int mode) {
   hipLaunchKernelGGL(rowptr_kernel, dim3(cdiv(md.rows + 1, 256)), dim3(256), 0, ctx->stream, md.row, nnz, md.rows,
                      md.rowptr);
   TENRSO_HIP_TRY(hipGetLastError());
+  if (nnz > 0 && t->nd >= 2) TENRSO_TRY(sp_build_fibres(ctx, t, mode));
   return TENRSO_OK;
 }
 
@@ -272,6 +352,9 @@ void sp_destroy(tenrso_sptensor* t) {
     if (md.row) (void)hipFree(md.row);
     if (md.oth) (void)hipFree(md.oth);
     if (md.vals) (void)hipFree(md.vals);
+    if (md.fstart) (void)hipFree(md.fstart);
+    if (md.fpre) (void)hipFree(md.fpre);
+    if (md.chunk_fib) (void)hipFree(md.chunk_fib);
   }
   if (t->idx) (void)hipFree(t->idx);
   if (t->vals) (void)hipFree(t->vals);
@@ -491,6 +574,137 @@ __global__ __launch_bounds__(256) void sp_mttkrp_vec_kernel(SpArgs a, float* __r
   }
 }
 
+// Fibre form (R % 4 == 0, 16-byte aligned factors, 2 <= nd <= 4): the same
+// worker / carry contract as sp_mttkrp_vec_kernel, reading the compressed
+// fibres instead of per-nonzero rows and prefixes.  Per pass of SP_SUB nonzeros
+// a worker stages the leaves (last other mode's index, value) and a window of
+// FWIN fibres starting at the fibre of the pass's first nonzero (fibre starts
+// and prefixes); each nonzero's fibre follows from a cursor that only moves
+// forward through the window.  Bytes per nonzero: 8 (leaf, value) plus 4 (D + 1)
+// per fibre, against 4 (D + 2) for the sorted COO.  D is a template parameter,
+// so the factor-row gathers of every other mode of a batch issue together.
+constexpr int SP_FWIN = SP_SUB + 2;  // fibres a pass can touch, plus the next start
+
+template <int G, int D>
+static constexpr size_t sp_csf_lds_bytes() {
+  return (size_t)4 * (64 / G) * (2 * SP_SUB + (D + 1) * SP_FWIN) * sizeof(int32_t);
+}
+
+struct SpCsfArgs {
+  i64 nnz, rows, R, nfib;
+  int r0, rcount;
+  const int32_t* leaf;     // last other mode's index per sorted nonzero
+  const float* vals;
+  const int32_t* fstart;   // nfib + 1
+  const int32_t* fpre;     // D x nfib
+  const int32_t* chunk_fib;
+  const float* F[MAXD_SP]; // factors of the other modes, ascending mode order
+  i64 nworkers;
+};
+
+template <int G, int D>
+__global__ __launch_bounds__(256) void sp_mttkrp_csf_kernel(SpCsfArgs a, float* __restrict__ out,
+                                                            float* __restrict__ cval, int32_t* __restrict__ crow) {
+  constexpr int NPW = 64 / G;      // workers per wave
+  constexpr int PER = SP_SUB / G;  // staged leaf entries per lane and array
+  constexpr int WSTRIDE = 2 * SP_SUB + (D + 1) * SP_FWIN;
+  static_assert(PER == 2 || PER % 4 == 0, "staging width");
+  extern __shared__ int32_t sp_stage[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = lane / G, q = lane % G;
+  int32_t* sLeaf = sp_stage + (wave * NPW + g) * WSTRIDE;
+  int32_t* sVal = sLeaf + SP_SUB;
+  int32_t* sFst = sVal + SP_SUB;       // SP_FWIN fibre starts
+  int32_t* sPre = sFst + SP_FWIN;      // D x SP_FWIN prefixes (row first)
+  const i64 w = ((i64)blockIdx.x * 4 + wave) * NPW + g;
+  const bool active = w < a.nworkers;
+  const i64 e0 = w * SP_CSF_W;
+  const i64 e1 = active ? ((e0 + SP_CSF_W) < a.nnz ? e0 + SP_CSF_W : a.nnz) : e0;
+  const bool live = 4 * q < a.rcount;
+  const int col = a.r0 + (live ? 4 * q : 0);
+  i64 flo = active ? a.chunk_fib[w] : 0;  // first fibre of the staged window
+  const int first = active ? a.fpre[flo] : 0;
+  int cur = first;
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  auto flush = [&](int row, const floatx4& v) {
+    if (!live) return;
+    float* dst = (row == first) ? cval + (2 * w) * a.R + col : out + (i64)row * a.R + col;
+    *reinterpret_cast<floatx4*>(dst) = v;
+  };
+  for (i64 base = e0; base < e1; base += SP_SUB) {
+    const int n = (int)((e1 - base) < SP_SUB ? (e1 - base) : SP_SUB);
+    sp_stage_part<PER>(a.leaf + base, sLeaf, q * PER, n);
+    sp_stage_part<PER>(reinterpret_cast<const int32_t*>(a.vals) + base, sVal, q * PER, n);
+    // fibre window: starts flo .. flo + SP_FWIN - 1 (past the last fibre: nnz)
+    // and prefixes (past the last fibre: the last fibre's, never used)
+    for (int j = q; j < SP_FWIN; j += G) {
+      const i64 f = flo + j;
+      sFst[j] = f <= a.nfib ? a.fstart[f] : (int32_t)a.nnz;
+      const i64 fc = f < a.nfib ? f : a.nfib - 1;
+#pragma unroll
+      for (int l = 0; l < D; ++l) sPre[l * SP_FWIN + j] = a.fpre[(i64)l * a.nfib + fc];
+    }
+    __builtin_amdgcn_wave_barrier();
+    int fc = 0;  // window index of the current nonzero's fibre
+    for (int i0 = 0; i0 < n; i0 += SP_VU) {
+      int rr[SP_VU];
+      floatx4 pv[SP_VU];
+      int oi[D][SP_VU];
+#pragma unroll
+      for (int u = 0; u < SP_VU; ++u) {
+        const i64 e = base + i0 + u;
+        // the cursor moves forward only; past the run's end it stays put
+        if (i0 + u < n)
+          while ((i64)sFst[fc + 1] <= e) ++fc;
+        rr[u] = sPre[fc];
+#pragma unroll
+        for (int l = 1; l < D; ++l) oi[l - 1][u] = sPre[l * SP_FWIN + fc];
+        // staged entries past n are zero (row 0 is a valid factor row)
+        oi[D - 1][u] = sLeaf[i0 + u];
+        const float v = __int_as_float(sVal[i0 + u]);
+        pv[u] = floatx4{v, v, v, v};
+      }
+      floatx4 fv[D][SP_VU];
+#pragma unroll
+      for (int l = 0; l < D; ++l)
+#pragma unroll
+        for (int u = 0; u < SP_VU; ++u)
+          fv[l][u] = *reinterpret_cast<const floatx4*>(a.F[l] + col + (i64)oi[l][u] * a.R);
+#pragma unroll
+      for (int l = 0; l < D; ++l)
+#pragma unroll
+        for (int u = 0; u < SP_VU; ++u) pv[u] *= fv[l][u];
+#pragma unroll
+      for (int u = 0; u < SP_VU; ++u) {
+        if (i0 + u >= n) break;  // past the run's end: must not switch rows
+        if (rr[u] != cur) {
+          flush(cur, acc);
+          cur = rr[u];
+          acc = floatx4{0.f, 0.f, 0.f, 0.f};
+        }
+        acc += pv[u];
+      }
+    }
+    flo += fc;  // the fibre of the pass's last nonzero starts the next window
+    __builtin_amdgcn_wave_barrier();  // staged entries read before the next pass overwrites them
+  }
+  if (!active) return;
+  if (live) {
+    float* c0 = cval + (2 * w) * a.R + col;
+    float* c1 = cval + (2 * w + 1) * a.R + col;
+    if (cur == first) {
+      *reinterpret_cast<floatx4*>(c0) = acc;
+      *reinterpret_cast<floatx4*>(c1) = floatx4{0.f, 0.f, 0.f, 0.f};
+    } else {
+      *reinterpret_cast<floatx4*>(c1) = acc;
+    }
+  }
+  if (q == 0) {
+    crow[2 * w] = first;
+    crow[2 * w + 1] = cur;
+  }
+}
+
 // Segmented reduction of sorted (key, R-vector) slots, one LPN-lane worker per
 // tile of SP_TILE slots: keys strictly inside a tile are complete and stored to
 // `out`; the tile's first and last keys go to the next level's slots (2 per
@@ -553,9 +767,25 @@ __global__ __launch_bounds__(256) void zero_f32_kernel(float* __restrict__ p, i6
 
 template <int LPN>
 static int sp_mttkrp_launch(tenrso_ctx* ctx, SpArgs& a, float* out, float* cv0, int32_t* ck0, float* cv1,
-                            int32_t* ck1, bool vec) {
+                            int32_t* ck1, bool vec, const SpCsfArgs* csf) {
   const i64 R = a.R;
-  if (vec) {
+  if (csf) {
+    constexpr int G = LPN / 4;
+    constexpr int NPW = 64 / G;
+    const int D = a.nd_oth;
+    SpCsfArgs ca = *csf;
+    ca.r0 = a.r0;
+    ca.rcount = a.rcount;
+    const dim3 grid(cdiv(ca.nworkers, 4 * NPW)), block(256);
+    constexpr size_t lds1 = sp_csf_lds_bytes<G, 1>(), lds2 = sp_csf_lds_bytes<G, 2>(),
+                     lds3 = sp_csf_lds_bytes<G, 3>();
+    if (D == 1)
+      hipLaunchKernelGGL((sp_mttkrp_csf_kernel<G, 1>), grid, block, lds1, ctx->stream, ca, out, cv0, ck0);
+    else if (D == 2)
+      hipLaunchKernelGGL((sp_mttkrp_csf_kernel<G, 2>), grid, block, lds2, ctx->stream, ca, out, cv0, ck0);
+    else
+      hipLaunchKernelGGL((sp_mttkrp_csf_kernel<G, 3>), grid, block, lds3, ctx->stream, ca, out, cv0, ck0);
+  } else if (vec) {
     constexpr int G = LPN / 4;
     constexpr int NPW = 64 / G;
     hipLaunchKernelGGL(sp_mttkrp_vec_kernel<G>, dim3(cdiv(a.nworkers, 4 * NPW)), dim3(256),
@@ -605,7 +835,23 @@ int sp_mttkrp(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* F, 
   // 16-byte aligned factor / output bases
   bool vec = (R % 4) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
   for (int k = 0; k < a.nd_oth; ++k) vec = vec && (reinterpret_cast<uintptr_t>(a.F[k]) & 15) == 0;
-  a.nworkers = vec ? (t->nnz + SP_W - 1) / SP_W : (t->nnz + SP_CH - 1) / SP_CH;
+  // the fibre form for 2- to 4-way tensors (vector gathers, fibres built)
+  const bool use_csf = vec && a.nd_oth >= 1 && a.nd_oth <= 3 && md.fstart != nullptr && md.nfib > 0;
+  SpCsfArgs ca{};
+  if (use_csf) {
+    ca.nnz = t->nnz;
+    ca.rows = I;
+    ca.R = R;
+    ca.nfib = md.nfib;
+    ca.leaf = md.oth + (i64)(a.nd_oth - 1) * t->nnz;
+    ca.vals = md.vals;
+    ca.fstart = md.fstart;
+    ca.fpre = md.fpre;
+    ca.chunk_fib = md.chunk_fib;
+    for (int k = 0; k < a.nd_oth; ++k) ca.F[k] = a.F[k];
+    ca.nworkers = md.nchunks;
+  }
+  a.nworkers = use_csf ? md.nchunks : (vec ? (t->nnz + SP_W - 1) / SP_W : (t->nnz + SP_CH - 1) / SP_CH);
   const i64 nslots = 2 * a.nworkers;
   void *pv, *pk;
   TENRSO_TRY(ctx_buffer(ctx, SLOT_TMP, (size_t)2 * nslots * R * sizeof(float), &pv));
@@ -619,9 +865,10 @@ int sp_mttkrp(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* F, 
   for (int rc = 0; rc < rchunks; ++rc) {
     a.r0 = rc * LPN;
     a.rcount = (int)std::min<i64>(LPN, R - a.r0);
-    if (LPN == 16) TENRSO_TRY(sp_mttkrp_launch<16>(ctx, a, out, cv0, ck0, cv1, ck1, vec));
-    else if (LPN == 32) TENRSO_TRY(sp_mttkrp_launch<32>(ctx, a, out, cv0, ck0, cv1, ck1, vec));
-    else TENRSO_TRY(sp_mttkrp_launch<64>(ctx, a, out, cv0, ck0, cv1, ck1, vec));
+    const SpCsfArgs* cp = use_csf ? &ca : nullptr;
+    if (LPN == 16) TENRSO_TRY(sp_mttkrp_launch<16>(ctx, a, out, cv0, ck0, cv1, ck1, vec, cp));
+    else if (LPN == 32) TENRSO_TRY(sp_mttkrp_launch<32>(ctx, a, out, cv0, ck0, cv1, ck1, vec, cp));
+    else TENRSO_TRY(sp_mttkrp_launch<64>(ctx, a, out, cv0, ck0, cv1, ck1, vec, cp));
   }
   return TENRSO_OK;
 }

@@ -14,7 +14,20 @@ struct SpMode {
   int32_t* row = nullptr;      // nnz: output-mode index of each sorted nonzero
   int32_t* oth = nullptr;      // (nd-1) x nnz: other modes' indices, ascending mode order
   float* vals = nullptr;       // nnz
+  // Compressed fibres (the CSF of mode order [n, others ascending] cut above its
+  // leaf level): fibre f holds the sorted nonzeros fstart[f] .. fstart[f+1] that
+  // share the prefix (i_n, i_o1, .., i_o(D-1)), D = nd - 1; fpre is D x nfib (SoA:
+  // row, then the prefix's other indices).  The leaves are the last other mode's
+  // indices (oth + (D-1) * nnz) and vals.  chunk_fib[c] is the fibre holding
+  // nonzero c * SP_CSF_W.
+  i64 nfib = 0;
+  int32_t* fstart = nullptr;   // nfib + 1
+  int32_t* fpre = nullptr;     // D x nfib
+  int32_t* chunk_fib = nullptr;
+  i64 nchunks = 0;
 };
+
+constexpr int SP_CSF_W = 512;  // nonzeros per worker of the fibre kernel
 
 }  // namespace tenrso
 

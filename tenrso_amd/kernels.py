@@ -23,7 +23,7 @@ from ._lib import check, lib
 __all__ = [
     "mttkrp", "mttkrp_blocked", "mttkrp_blocked_parallel", "mttkrp_fused", "mttkrp_fused_parallel",
     "mttkrp_all_modes", "mttkrp_bf16", "to_bf16", "khatri_rao", "khatri_rao_parallel", "khatri_rao_blocked",
-    "khatri_rao_blocked_parallel", "cp_reconstruct", "extract_weights",
+    "khatri_rao_blocked_parallel", "cp_reconstruct", "extract_weights", "mttkrp_streaming",
 ]
 
 
@@ -282,3 +282,43 @@ def extract_weights(factors):
     if _is_torch(factors[0]):
         return [f.cpu() for f in F], w.cpu()
     return [f.cpu().numpy() for f in F], w.cpu().numpy()
+
+
+def mttkrp_streaming(tensor, factors, mode: int, device_budget_bytes: int, return_stats: bool = False):
+    """Out-of-core MTTKRP (the streaming MTTKRP that tenrso-ooc's example
+    mttkrp_streaming.rs:66-110 describes and leaves as a call of the in-memory
+    mttkrp): the tensor stays in HOST memory (a numpy array or a torch CPU tensor,
+    pinned or pageable) and streams through at most ``device_budget_bytes`` of
+    device memory in mode-0 slabs, double-buffered against the MFMA MTTKRP of the
+    previous slab.  Returns the (I_mode, R) result on the host (numpy, or torch
+    CPU for a torch input) and, with ``return_stats``, a dict: seconds, h2d_GBs
+    (tensor bytes moved per second), slab_rows, slabs, pinned (the input was
+    pinned), host_copy_s (time spent staging a pageable input)."""
+    shape = tuple(int(s) for s in tensor.shape)
+    R = _check_mttkrp(shape, factors, mode)
+    is_torch = _is_torch(tensor)
+    if is_torch and tensor.is_cuda:
+        raise TypeError("mttkrp_streaming takes a host tensor (use mttkrp for device tensors)")
+    if is_torch:
+        import torch
+        X = tensor.detach()
+        if X.dtype != torch.float32 or not X.is_contiguous():
+            X = X.to(torch.float32).contiguous()
+        xptr = X.data_ptr()
+    else:
+        X = _np32(tensor)
+        xptr = X.ctypes.data
+    F = [_np32(_to_numpy(f)) for f in factors]
+    out = np.empty((shape[mode], R), dtype=np.float32)
+    stats = np.zeros(8, dtype=np.float64)
+    ctx = _lib.context_for(0)
+    ptrs = (C.c_void_p * len(F))(*[f.ctypes.data for f in F])
+    check(lib().tenrso_mttkrp_f32_streaming(ctx.ptr, xptr, len(shape), _i64s(shape), ptrs, R, int(mode),
+                                            out.ctypes.data, int(device_budget_bytes), stats.ctypes.data))
+    if is_torch:
+        import torch
+        out = torch.from_numpy(out)
+    if not return_stats:
+        return out
+    return out, {"seconds": float(stats[0]), "h2d_GBs": float(stats[1]), "slab_rows": int(stats[2]),
+                 "slabs": int(stats[3]), "pinned": bool(stats[4]), "host_copy_s": float(stats[5])}

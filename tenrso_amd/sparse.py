@@ -330,6 +330,13 @@ class SparseTensor:
                                     loc.ctypes.data, vals.ctypes.data))
         return HiCooTensor(self.shape, block_shape, bc, bp, loc, vals)
 
+    def fibres(self, mode):
+        """Number of compressed fibres of the mode (distinct (i_mode, i_o1, .., i_o(N-2))
+        prefixes), the level above the leaves that the sparse MTTKRP reads."""
+        n = C.c_int64()
+        check(lib().tenrso_sp_fibres(self.ptr, int(mode), C.byref(n)))
+        return int(n.value)
+
     def csf(self, mode):
         """CsfTensor of mode order [mode, others ascending], built from the device sort."""
         nd = len(self.shape)
