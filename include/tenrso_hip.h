@@ -152,6 +152,29 @@ int tenrso_mttkrp_all_modes_f32(tenrso_ctx* ctx, const float* X, int ndim, const
                                 const float* const* factors, int64_t rank, float* const* outs);
 
 /* ------------------------------------------------------------------------
+ * n-mode products (TTM).  Replaces tenrso_kernels::nmode_product
+ * (tenrso-kernels/src/nmode.rs:72-130): for X of shape (I_0..I_{N-1}) and a
+ * row-major matrix M of rows x cols with cols == I_mode, Y = X x_mode M has
+ * shape (I_0, .., rows, .., I_{N-1}), Y[.., j, ..] = sum_i M[j, i] X[.., i, ..].
+ * All pointers are device pointers; the product is a batched FP32 GEMM
+ * (rocBLAS) on the context's stream.  The reference's checks and messages:
+ * "Mode %d out of bounds for tensor with rank %d", "Matrix columns (%d) must
+ * match tensor mode-%d size (%d)" (TENRSO_ERR_INVALID_ARGUMENT).
+ * ------------------------------------------------------------------------ */
+int tenrso_nmode_product_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,
+                             const float* matrix, int64_t rows, int64_t cols, int mode, float* Y);
+/* nmode_products_seq (nmode.rs:241-255) and tucker_operator_ordered
+ * (nmode.rs:517-530): the products (matrices[t], modes[t]) applied in order,
+ * each checked against the shape the previous ones left; intermediates live in
+ * the context's buffers and the last product writes Y, whose shape is returned
+ * in out_shape (ndim entries).  Tucker-HOOI's TTM chain
+ * (tenrso-decomp/src/tucker.rs:1122-1140) is this call with the transposed
+ * factors of every mode but one. */
+int tenrso_nmode_products_seq_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, int nmat,
+                                  const float* const* matrices, const int64_t* rows, const int64_t* cols,
+                                  const int* modes, float* Y, int64_t* out_shape);
+
+/* ------------------------------------------------------------------------
  * Khatri-Rao product.  Replaces tenrso_kernels::khatri_rao (khatri_rao.rs:56-89)
  * and its _parallel / _blocked / _blocked_parallel variants (117-357), which are
  * bit-identical: C[ia*J + ib, k] = A[ia, k] * B[ib, k].  The reference panics on

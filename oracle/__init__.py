@@ -32,6 +32,9 @@ Parity status of each function (how it is pinned, see tests/test_oracle.py):
                                        plain run bit for bit.  orthogonal=True uses
                                        Householder QR: parity unpinned.
   cp_reconstruct / extract_weights     pinned: numpy einsum, KAT outer.rs doc test.
+  nmode_product / nmode_products_seq   pinned: nmode.rs tests (mode-0 exact values,
+                                       identity, shapes, the error messages) and a
+                                       brute-force loop on small shapes.
   cp_factor_grads                      pinned: FP64 central differences, identity
                                        dL/dA_n = A_n G_n - M_n.
   coo_mttkrp                           pinned: COO -> dense + dense MTTKRP.

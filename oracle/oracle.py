@@ -10,7 +10,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 __all__ = [
     "lib", "mttkrp", "mttkrp_fast", "mttkrp_blocked_parallel", "khatri_rao", "gram", "gram_hadamard",
     "solve_ls", "tikhonov", "soft_threshold", "orthonormalize", "norm_sq", "recon_norm_sq", "fit",
-    "cp_als", "cp_reconstruct", "extract_weights", "cp_factor_grads", "coo_mttkrp", "num_threads", "svd_init",
+    "cp_als", "cp_reconstruct", "extract_weights", "nmode_product", "nmode_products_seq", "cp_factor_grads", "coo_mttkrp", "num_threads", "svd_init",
     "hicoo_from_coo",
     "OrcConstraints", "REASONS", "LinalgError", "F32_EPS",
 ]
@@ -276,6 +276,27 @@ def extract_weights(factors, eps=float(np.finfo(np.float32).eps)):
     w = np.zeros(R)
     lib().orc_extract_weights_f64(_ptrs(F), len(F), _shape(shape), C.c_int64(R), _p(w), C.c_double(eps))
     return F, w
+
+
+def nmode_product(X, M, mode):
+    """nmode_product (tenrso-kernels/src/nmode.rs:72-130) in FP64: Y = X x_mode M,
+    Y[.., j, ..] = sum_i M[j, i] X[.., i, ..], with the reference's checks and
+    messages (:82-98).  numpy tensordot over the mode, moved back into place."""
+    X = np.asarray(X, dtype=np.float64)
+    M = np.asarray(M, dtype=np.float64)
+    if mode >= X.ndim:
+        raise ValueError(f"Mode {mode} out of bounds for tensor with rank {X.ndim}")
+    if M.shape[1] != X.shape[mode]:
+        raise ValueError(f"Matrix columns ({M.shape[1]}) must match tensor mode-{mode} size ({X.shape[mode]})")
+    return np.moveaxis(np.tensordot(M, X, axes=(1, mode)), 0, mode)
+
+
+def nmode_products_seq(X, pairs):
+    """nmode_products_seq (nmode.rs:241-255): the products (matrix, mode) in order."""
+    Y = np.asarray(X, dtype=np.float64)
+    for M, mode in pairs:
+        Y = nmode_product(Y, M, mode)
+    return Y
 
 
 def cp_factor_grads(grad_output, factors, weights=None):

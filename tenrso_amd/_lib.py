@@ -65,6 +65,8 @@ SIGNATURES = {
     "tenrso_mttkrp_f32_noalloc": ([_vp, _vp, _i, _vp, _vp, _i64, _i, _vp], _i),
     "tenrso_mttkrp_all_modes_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _vp], _i),
     "tenrso_mttkrp_f32_streaming": ([_vp, _vp, _i, _vp, _vp, _i64, _i, _vp, _i64, _vp], _i),
+    "tenrso_nmode_product_f32": ([_vp, _vp, _i, _vp, _vp, _i64, _i64, _i, _vp], _i),
+    "tenrso_nmode_products_seq_f32": ([_vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "tenrso_khatri_rao_f32": ([_vp, _vp, _i64, _vp, _i64, _i64, _vp], _i),
     "tenrso_khatri_rao_f32_host": ([_vp, _vp, _i64, _vp, _i64, _i64, _vp], _i),
     "tenrso_cp_reconstruct_f32": ([_vp, _i, _vp, _vp, _i64, _vp, _vp], _i),

@@ -69,6 +69,7 @@ struct tenrso_ctx {
   const void* occ_kern[32] = {};
   int occ_val[32] = {};
   int occ_n = 0;
+  void* blas = nullptr;              // rocBLAS handle of the n-mode products, made on first use
 };
 
 namespace tenrso {
@@ -100,6 +101,9 @@ enum BufSlot : int {
 };
 
 int ctx_buffer(tenrso_ctx* ctx, int slot, size_t bytes, void** out);
+
+// releases the context's rocBLAS handle (nmode.hip)
+void blas_destroy(tenrso_ctx* ctx);
 
 inline int set_device(tenrso_ctx* ctx) {
   TENRSO_HIP_TRY(hipSetDevice(ctx->device));

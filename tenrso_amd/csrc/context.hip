@@ -121,6 +121,7 @@ int tenrso_ctx_destroy(tenrso_ctx* c) {
   if (!c) return TENRSO_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  tenrso::blas_destroy(c);
   for (auto& b : c->bufs)
     if (b.ptr) (void)hipFree(b.ptr);
   if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -24,6 +24,7 @@ __all__ = [
     "mttkrp", "mttkrp_blocked", "mttkrp_blocked_parallel", "mttkrp_fused", "mttkrp_fused_parallel",
     "mttkrp_all_modes", "mttkrp_bf16", "to_bf16", "khatri_rao", "khatri_rao_parallel", "khatri_rao_blocked",
     "khatri_rao_blocked_parallel", "cp_reconstruct", "extract_weights", "mttkrp_streaming",
+    "nmode_product", "nmode_products_seq", "tucker_operator", "tucker_operator_ordered", "tucker_reconstruct",
 ]
 
 
@@ -322,3 +323,85 @@ def mttkrp_streaming(tensor, factors, mode: int, device_budget_bytes: int, retur
         return out
     return out, {"seconds": float(stats[0]), "h2d_GBs": float(stats[1]), "slab_rows": int(stats[2]),
                  "slabs": int(stats[3]), "pinned": bool(stats[4]), "host_copy_s": float(stats[5])}
+
+
+# ---------------------------------------------------------------- n-mode products
+def _ttm_chain(tensor, mats, modes):
+    """Y = tensor x_{modes[0]} mats[0] x_{modes[1]} mats[1] ... in one C call
+    (tenrso_nmode_products_seq_f32: batched FP32 GEMMs on the device, the
+    intermediates in the context's buffers).  Checks every product against the
+    shape the previous ones leave, with the messages of nmode.rs:82-98.  A CUDA
+    tensor stays on its device (torch's current stream); a host tensor is copied
+    to the current device and the result back."""
+    import torch
+    shape = [int(s) for s in tensor.shape]
+    nd = len(shape)
+    cur = list(shape)
+    for M, m in zip(mats, modes):
+        if m < 0 or m >= nd:
+            raise ValueError(f"Mode {m} out of bounds for tensor with rank {nd}")
+        if int(M.shape[1]) != cur[m]:
+            raise ValueError(f"Matrix columns ({int(M.shape[1])}) must match tensor mode-{m} size ({cur[m]})")
+        cur[m] = int(M.shape[0])
+    on_dev = _is_cuda(tensor)
+    dev = tensor.device if on_dev else torch.device("cuda", torch.cuda.current_device())
+    X = _dev_f32(tensor, dev)
+    Ms = [_dev_f32(M, dev) for M in mats]
+    Y = torch.empty(cur, dtype=torch.float32, device=dev)
+    ctx = _lib.torch_context(X)
+    n = len(Ms)
+    ptrs = (C.c_void_p * max(n, 1))(*[M.data_ptr() for M in Ms])
+    out_shape = (C.c_int64 * max(nd, 1))()
+    check(lib().tenrso_nmode_products_seq_f32(
+        ctx.ptr, X.data_ptr(), nd, _i64s(shape), n, ptrs, _i64s([M.shape[0] for M in Ms] or [0]),
+        _i64s([M.shape[1] for M in Ms] or [0]), (C.c_int * max(n, 1))(*[int(m) for m in modes]),
+        Y.data_ptr(), out_shape))
+    if list(out_shape)[:nd] != cur:
+        raise RuntimeError(f"n-mode chain shape {list(out_shape)[:nd]} != {cur}")
+    if on_dev:
+        return Y
+    out = Y.cpu()
+    return out if _is_torch(tensor) else out.numpy()
+
+
+def nmode_product(tensor, matrix, mode: int):
+    """nmode_product (nmode.rs:72-130): Y = X x_mode M, M of shape (J, I_mode),
+    Y[.., j, ..] = sum_i M[j, i] X[.., i, ..] (FP32 GEMM on the device)."""
+    return _ttm_chain(tensor, [matrix], [int(mode)])
+
+
+def nmode_products_seq(tensor, matrices):
+    """nmode_products_seq (nmode.rs:241-255): ``matrices`` is a sequence of
+    (matrix, mode) pairs applied in order."""
+    return _ttm_chain(tensor, [m for m, _ in matrices], [int(k) for _, k in matrices])
+
+
+def tucker_operator_ordered(tensor, mode_matrix_pairs):
+    """tucker_operator_ordered (nmode.rs:517-530): (mode, matrix) pairs in order."""
+    return _ttm_chain(tensor, [m for _, m in mode_matrix_pairs], [int(k) for k, _ in mode_matrix_pairs])
+
+
+def tucker_operator(tensor, factor_matrices: dict):
+    """tucker_operator (nmode.rs:442-485): the products of a {mode: matrix} map,
+    modes checked first, then applied in ascending order of the ratio
+    rows / I_mode (the most shrinking first).  The reference's ties follow its
+    hash map's order; here ties go by ascending mode."""
+    nd = len(tensor.shape)
+    for m in factor_matrices:
+        if m < 0 or m >= nd:
+            raise ValueError(f"Mode {m} out of bounds for tensor with rank {nd}")
+    order = sorted(factor_matrices, key=lambda m: (int(factor_matrices[m].shape[0]) / int(tensor.shape[m]), m))
+    return _ttm_chain(tensor, [factor_matrices[m] for m in order], order)
+
+
+def tucker_reconstruct(core, factors):
+    """tucker_reconstruct (nmode.rs:576-612): X = G x_0 U_0 x_1 U_1 ... through
+    tucker_operator, with the reference's checks and messages."""
+    nd = len(core.shape)
+    if len(factors) != nd:
+        raise ValueError(f"Number of factor matrices ({len(factors)}) must match core rank ({nd})")
+    for i, f in enumerate(factors):
+        if int(f.shape[1]) != int(core.shape[i]):
+            raise ValueError(f"Factor matrix {i} has {int(f.shape[1])} columns, expected {int(core.shape[i])} "
+                             f"(core mode-{i} size)")
+    return tucker_operator(core, dict(enumerate(factors)))

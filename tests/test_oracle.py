@@ -500,3 +500,33 @@ def test_hicoo_from_coo_invariants(orc):
     for b in range(bc.shape[1]):
         seg = v[bp[b]:bp[b + 1]]
         assert (np.diff(seg) > 0).all()
+
+
+def test_nmode_product_reference_cases(orc):
+    """nmode.rs tests: mode-0 exact values (2x3 by a 3x2 matrix), identity, the
+    shape cases, the error messages; plus a brute-force loop."""
+    T = np.array([[1.0, 2.0, 3.0], [4.0, 5.0, 6.0]])
+    M = np.array([[1.0, 0.0], [0.0, 1.0], [1.0, 1.0]])
+    assert np.array_equal(orc.nmode_product(T, M, 0), [[1, 2, 3], [4, 5, 6], [5, 7, 9]])
+    X = np.arange(24, dtype=np.float64).reshape(2, 3, 4)
+    assert np.array_equal(orc.nmode_product(X, np.eye(3), 1), X)
+    M5 = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [0, 1, 1]], np.float64)
+    assert orc.nmode_product(X, M5, 1).shape == (2, 5, 4)
+    m1, m2 = np.eye(2), np.array([[1.0, 0, 0], [0, 1.0, 0]])
+    assert orc.nmode_products_seq(X, [(m1, 0), (m2, 1)]).shape == (2, 2, 4)
+    with pytest.raises(ValueError, match="Mode 5 out of bounds"):
+        orc.nmode_product(np.ones((2, 3)), np.ones((1, 2)), 5)
+    with pytest.raises(ValueError, match=r"Matrix columns \(2\) must match tensor mode-1 size \(3\)"):
+        orc.nmode_product(np.ones((2, 3)), np.ones((1, 2)), 1)
+    rng = np.random.default_rng(31)
+    X = rng.standard_normal((3, 4, 5))
+    for mode in range(3):
+        M = rng.standard_normal((2, X.shape[mode]))
+        Y = orc.nmode_product(X, M, mode)
+        ref = np.zeros(Y.shape)
+        for idx in np.ndindex(*Y.shape):
+            for i in range(X.shape[mode]):
+                src = list(idx)
+                src[mode] = i
+                ref[idx] += M[idx[mode], i] * X[tuple(src)]
+        assert np.allclose(Y, ref, rtol=1e-13, atol=1e-13)
