@@ -250,7 +250,7 @@ def extra_configs(with_cpu):
         fac = sum(shape[k] * R * 4 for k in range(3) if k != mode) + shape[mode] * R * 4
         alg = nnz * (3 * 4 + 4) + fac  # the sorted COO form (row, two indices, value per nonzero)
         nfib = sp.fibres(mode)
-        csf = nnz * 8 + nfib * 4 * 3 + fac  # what the fibre kernel reads: leaves, fibre starts and prefixes
+        csf = nnz * 8 + nfib * 4 * 2 + fac  # what the fibre kernel reads: leaves and values, fibre prefixes
         modes[str(mode)] = {"ms": t, "GBs": alg / (t * 1e-3) / 1e9, "algorithmic_bytes": alg,
                             "fibres": nfib, "csf_bytes": csf, "csf_GBs": csf / (t * 1e-3) / 1e9,
                             "csf_frac_of_hbm": csf / (t * 1e-3) / 1e9 / PEAK_HBM_GBS}

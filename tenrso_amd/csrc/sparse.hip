@@ -120,21 +120,22 @@ __global__ __launch_bounds__(256) void fib_flag_kernel(const int32_t* __restrict
   flag[e] = f;
 }
 
-// pos = exclusive scan of flag: fibre f = pos[e] starts at e where flag[e]
+// pos = exclusive scan of flag: fibre f = pos[e] starts at e where flag[e].
+// Scatters the prefixes and writes the leaves with their fibre-end bit.
 __global__ __launch_bounds__(256) void fib_scatter_kernel(const int32_t* __restrict__ flag,
                                                           const int32_t* __restrict__ pos,
                                                           const int32_t* __restrict__ row,
                                                           const int32_t* __restrict__ oth, i64 nnz, int D, i64 nfib,
-                                                          int32_t* __restrict__ fstart, int32_t* __restrict__ fpre) {
+                                                          int32_t* __restrict__ leafe, int32_t* __restrict__ fpre) {
   const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
   if (e >= nnz) return;
   if (flag[e]) {
     const i64 f = pos[e];
-    fstart[f] = (int32_t)e;
     fpre[f] = row[e];
     for (int l = 1; l < D; ++l) fpre[(i64)l * nfib + f] = oth[(i64)(l - 1) * nnz + e];
   }
-  if (e == nnz - 1) fstart[nfib] = (int32_t)nnz;
+  const bool last = e == nnz - 1 || flag[e + 1];
+  leafe[e] = (int32_t)((uint32_t)oth[(i64)(D - 1) * nnz + e] | (last ? 0x80000000u : 0u));
 }
 
 // chunk_fib[c] = the fibre holding nonzero c * W (inclusive scan - 1)
@@ -186,11 +187,11 @@ static int sp_build_fibres(tenrso_ctx* ctx, tenrso_sptensor* t, int mode) {
   TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));
   md.nfib = (i64)last[0] + last[1];
   md.nchunks = (nnz + SP_CSF_W - 1) / SP_CSF_W;
-  TENRSO_HIP_TRY(hipMalloc(&md.fstart, (size_t)(md.nfib + 1) * 4));
+  TENRSO_HIP_TRY(hipMalloc(&md.leafe, (size_t)nnz * 4));
   TENRSO_HIP_TRY(hipMalloc(&md.fpre, (size_t)std::max<i64>(1, md.nfib * D) * 4));
   TENRSO_HIP_TRY(hipMalloc(&md.chunk_fib, (size_t)md.nchunks * 4));
   hipLaunchKernelGGL(fib_scatter_kernel, dim3(cdiv(nnz, 256)), dim3(256), 0, ctx->stream, flag, pos, md.row, md.oth,
-                     nnz, D, md.nfib, md.fstart, md.fpre);
+                     nnz, D, md.nfib, md.leafe, md.fpre);
   hipLaunchKernelGGL(chunk_fib_kernel, dim3(cdiv(md.nchunks, 256)), dim3(256), 0, ctx->stream, flag, pos, md.nchunks,
                      SP_CSF_W, md.chunk_fib);
   TENRSO_HIP_TRY(hipGetLastError());
@@ -352,7 +353,7 @@ void sp_destroy(tenrso_sptensor* t) {
     if (md.row) (void)hipFree(md.row);
     if (md.oth) (void)hipFree(md.oth);
     if (md.vals) (void)hipFree(md.vals);
-    if (md.fstart) (void)hipFree(md.fstart);
+    if (md.leafe) (void)hipFree(md.leafe);
     if (md.fpre) (void)hipFree(md.fpre);
     if (md.chunk_fib) (void)hipFree(md.chunk_fib);
   }
@@ -575,27 +576,31 @@ __global__ __launch_bounds__(256) void sp_mttkrp_vec_kernel(SpArgs a, float* __r
 }
 
 // Fibre form (R % 4 == 0, 16-byte aligned factors, 2 <= nd <= 4): the same
-// worker / carry contract as sp_mttkrp_vec_kernel, reading the compressed
-// fibres instead of per-nonzero rows and prefixes.  Per pass of SP_SUB nonzeros
-// a worker stages the leaves (last other mode's index, value) and a window of
-// FWIN fibres starting at the fibre of the pass's first nonzero (fibre starts
-// and prefixes); each nonzero's fibre follows from a cursor that only moves
-// forward through the window.  Bytes per nonzero: 8 (leaf, value) plus 4 (D + 1)
-// per fibre, against 4 (D + 2) for the sorted COO.  D is a template parameter,
-// so the factor-row gathers of every other mode of a batch issue together.
-constexpr int SP_FWIN = SP_SUB + 2;  // fibres a pass can touch, plus the next start
+// worker / carry contract as sp_mttkrp_vec_kernel over the compressed fibres.
+// Per nonzero: the leaf (last other mode's index with the fibre-end bit) and
+// value, one factor-row gather, and v * F_last[leaf] added to the fibre's sum;
+// per fibre, at its last nonzero (or the run's last): the sum times the factor
+// rows of the prefix's middle indices (D - 1 gathers) added to the row.  So the
+// middle factors are gathered once per fibre instead of once per nonzero, and
+// the row / middle indices are read once per fibre.  A worker's fibre prefixes
+// are staged SP_FWIN at a time into LDS when the batch's fibres pass the window;
+// the fibre of each nonzero is the chunk's first fibre plus the ends before it.
+// Window: 64 fibres per worker, 32 where a wave holds 16 workers (LDS).
+template <int G>
+static constexpr int sp_fwin() {
+  return G >= 8 ? 64 : 32;
+}
 
 template <int G, int D>
 static constexpr size_t sp_csf_lds_bytes() {
-  return (size_t)4 * (64 / G) * (2 * SP_SUB + (D + 1) * SP_FWIN) * sizeof(int32_t);
+  return (size_t)4 * (64 / G) * (2 * SP_SUB + D * sp_fwin<G>()) * sizeof(int32_t);
 }
 
 struct SpCsfArgs {
   i64 nnz, rows, R, nfib;
   int r0, rcount;
-  const int32_t* leaf;     // last other mode's index per sorted nonzero
+  const int32_t* leafe;    // leaf index | fibre-end bit, per sorted nonzero
   const float* vals;
-  const int32_t* fstart;   // nfib + 1
   const int32_t* fpre;     // D x nfib
   const int32_t* chunk_fib;
   const float* F[MAXD_SP]; // factors of the other modes, ascending mode order
@@ -607,85 +612,101 @@ __global__ __launch_bounds__(256) void sp_mttkrp_csf_kernel(SpCsfArgs a, float* 
                                                             float* __restrict__ cval, int32_t* __restrict__ crow) {
   constexpr int NPW = 64 / G;      // workers per wave
   constexpr int PER = SP_SUB / G;  // staged leaf entries per lane and array
-  constexpr int WSTRIDE = 2 * SP_SUB + (D + 1) * SP_FWIN;
+  constexpr int SP_FWIN = sp_fwin<G>();
+  constexpr int WSTRIDE = 2 * SP_SUB + D * SP_FWIN;
   static_assert(PER == 2 || PER % 4 == 0, "staging width");
+  static_assert(SP_FWIN >= 2 * SP_VU, "fibre window");
   extern __shared__ int32_t sp_stage[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int g = lane / G, q = lane % G;
   int32_t* sLeaf = sp_stage + (wave * NPW + g) * WSTRIDE;
   int32_t* sVal = sLeaf + SP_SUB;
-  int32_t* sFst = sVal + SP_SUB;       // SP_FWIN fibre starts
-  int32_t* sPre = sFst + SP_FWIN;      // D x SP_FWIN prefixes (row first)
+  int32_t* sPre = sVal + SP_SUB;  // D x SP_FWIN prefixes (row first)
   const i64 w = ((i64)blockIdx.x * 4 + wave) * NPW + g;
   const bool active = w < a.nworkers;
   const i64 e0 = w * SP_CSF_W;
   const i64 e1 = active ? ((e0 + SP_CSF_W) < a.nnz ? e0 + SP_CSF_W : a.nnz) : e0;
   const bool live = 4 * q < a.rcount;
   const int col = a.r0 + (live ? 4 * q : 0);
-  i64 flo = active ? a.chunk_fib[w] : 0;  // first fibre of the staged window
-  const int first = active ? a.fpre[flo] : 0;
+  i64 fb = active ? a.chunk_fib[w] : 0;  // fibre of the next nonzero
+  i64 wlo = fb;                           // first fibre of the staged window
+  const int first = active ? a.fpre[fb] : 0;
   int cur = first;
-  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};   // the current row's sum
+  floatx4 facc = {0.f, 0.f, 0.f, 0.f};  // the current fibre's leaf sum
   auto flush = [&](int row, const floatx4& v) {
     if (!live) return;
     float* dst = (row == first) ? cval + (2 * w) * a.R + col : out + (i64)row * a.R + col;
     *reinterpret_cast<floatx4*>(dst) = v;
   };
+  auto stage_window = [&]() {
+    for (int j = q; j < SP_FWIN; j += G) {
+      const i64 f = wlo + j < a.nfib ? wlo + j : a.nfib - 1;  // past the last fibre: never used
+#pragma unroll
+      for (int l = 0; l < D; ++l) sPre[l * SP_FWIN + j] = a.fpre[(i64)l * a.nfib + f];
+    }
+  };
+  if (active) stage_window();
   for (i64 base = e0; base < e1; base += SP_SUB) {
     const int n = (int)((e1 - base) < SP_SUB ? (e1 - base) : SP_SUB);
-    sp_stage_part<PER>(a.leaf + base, sLeaf, q * PER, n);
+    sp_stage_part<PER>(a.leafe + base, sLeaf, q * PER, n);
     sp_stage_part<PER>(reinterpret_cast<const int32_t*>(a.vals) + base, sVal, q * PER, n);
-    // fibre window: starts flo .. flo + SP_FWIN - 1 (past the last fibre: nnz)
-    // and prefixes (past the last fibre: the last fibre's, never used)
-    for (int j = q; j < SP_FWIN; j += G) {
-      const i64 f = flo + j;
-      sFst[j] = f <= a.nfib ? a.fstart[f] : (int32_t)a.nnz;
-      const i64 fc = f < a.nfib ? f : a.nfib - 1;
-#pragma unroll
-      for (int l = 0; l < D; ++l) sPre[l * SP_FWIN + j] = a.fpre[(i64)l * a.nfib + fc];
-    }
     __builtin_amdgcn_wave_barrier();
-    int fc = 0;  // window index of the current nonzero's fibre
     for (int i0 = 0; i0 < n; i0 += SP_VU) {
-      int rr[SP_VU];
-      floatx4 pv[SP_VU];
-      int oi[D][SP_VU];
-#pragma unroll
-      for (int u = 0; u < SP_VU; ++u) {
-        const i64 e = base + i0 + u;
-        // the cursor moves forward only; past the run's end it stays put
-        if (i0 + u < n)
-          while ((i64)sFst[fc + 1] <= e) ++fc;
-        rr[u] = sPre[fc];
-#pragma unroll
-        for (int l = 1; l < D; ++l) oi[l - 1][u] = sPre[l * SP_FWIN + fc];
-        // staged entries past n are zero (row 0 is a valid factor row)
-        oi[D - 1][u] = sLeaf[i0 + u];
-        const float v = __int_as_float(sVal[i0 + u]);
-        pv[u] = floatx4{v, v, v, v};
+      // a batch touches fibres fb .. fb + SP_VU - 1: keep them in the window
+      if (fb + SP_VU > wlo + SP_FWIN) {
+        wlo = fb;
+        __builtin_amdgcn_wave_barrier();  // earlier window reads issued before it is overwritten
+        stage_window();
+        __builtin_amdgcn_wave_barrier();
       }
-      floatx4 fv[D][SP_VU];
-#pragma unroll
-      for (int l = 0; l < D; ++l)
-#pragma unroll
-        for (int u = 0; u < SP_VU; ++u)
-          fv[l][u] = *reinterpret_cast<const floatx4*>(a.F[l] + col + (i64)oi[l][u] * a.R);
-#pragma unroll
-      for (int l = 0; l < D; ++l)
-#pragma unroll
-        for (int u = 0; u < SP_VU; ++u) pv[u] *= fv[l][u];
+      floatx4 fl[SP_VU];
+      float v[SP_VU];
+      int fw[SP_VU];     // window index of the nonzero's fibre
+      bool done[SP_VU];  // the nonzero closes its fibre's contribution
+      int ends = 0;
 #pragma unroll
       for (int u = 0; u < SP_VU; ++u) {
-        if (i0 + u >= n) break;  // past the run's end: must not switch rows
-        if (rr[u] != cur) {
-          flush(cur, acc);
-          cur = rr[u];
-          acc = floatx4{0.f, 0.f, 0.f, 0.f};
+        const int32_t le = sLeaf[i0 + u];  // staged entries past n are zero: leaf 0, no end
+        const bool in = i0 + u < n;
+        fw[u] = (int)(fb - wlo) + ends;
+        done[u] = in && (le < 0 || base + i0 + u == e1 - 1);
+        ends += le < 0 ? 1 : 0;
+        v[u] = __int_as_float(sVal[i0 + u]);
+        fl[u] = *reinterpret_cast<const floatx4*>(a.F[D - 1] + col + (i64)(le & 0x7fffffff) * a.R);
+      }
+      floatx4 mid[D > 1 ? D - 1 : 1][SP_VU];
+      if constexpr (D > 1) {
+#pragma unroll
+        for (int l = 1; l < D; ++l)
+#pragma unroll
+          for (int u = 0; u < SP_VU; ++u)
+            mid[l - 1][u] = done[u] ? *reinterpret_cast<const floatx4*>(a.F[l - 1] + col +
+                                                                        (i64)sPre[l * SP_FWIN + fw[u]] * a.R)
+                                    : floatx4{0.f, 0.f, 0.f, 0.f};
+      }
+#pragma unroll
+      for (int u = 0; u < SP_VU; ++u) {
+        if (i0 + u >= n) break;  // past the run's end
+        facc += v[u] * fl[u];
+        if (done[u]) {
+          floatx4 t = facc;
+          if constexpr (D > 1) {
+#pragma unroll
+            for (int l = 0; l < D - 1; ++l) t *= mid[l][u];
+          }
+          const int r = sPre[fw[u]];
+          if (r != cur) {
+            flush(cur, acc);
+            cur = r;
+            acc = floatx4{0.f, 0.f, 0.f, 0.f};
+          }
+          acc += t;
+          facc = floatx4{0.f, 0.f, 0.f, 0.f};
         }
-        acc += pv[u];
       }
+      fb += ends;
     }
-    flo += fc;  // the fibre of the pass's last nonzero starts the next window
     __builtin_amdgcn_wave_barrier();  // staged entries read before the next pass overwrites them
   }
   if (!active) return;
@@ -836,16 +857,15 @@ int sp_mttkrp(tenrso_ctx* ctx, const tenrso_sptensor* t, const float* const* F, 
   bool vec = (R % 4) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
   for (int k = 0; k < a.nd_oth; ++k) vec = vec && (reinterpret_cast<uintptr_t>(a.F[k]) & 15) == 0;
   // the fibre form for 2- to 4-way tensors (vector gathers, fibres built)
-  const bool use_csf = vec && a.nd_oth >= 1 && a.nd_oth <= 3 && md.fstart != nullptr && md.nfib > 0;
+  const bool use_csf = vec && a.nd_oth >= 1 && a.nd_oth <= 3 && md.leafe != nullptr && md.nfib > 0;
   SpCsfArgs ca{};
   if (use_csf) {
     ca.nnz = t->nnz;
     ca.rows = I;
     ca.R = R;
     ca.nfib = md.nfib;
-    ca.leaf = md.oth + (i64)(a.nd_oth - 1) * t->nnz;
+    ca.leafe = md.leafe;
     ca.vals = md.vals;
-    ca.fstart = md.fstart;
     ca.fpre = md.fpre;
     ca.chunk_fib = md.chunk_fib;
     for (int k = 0; k < a.nd_oth; ++k) ca.F[k] = a.F[k];

@@ -15,13 +15,13 @@ struct SpMode {
   int32_t* oth = nullptr;      // (nd-1) x nnz: other modes' indices, ascending mode order
   float* vals = nullptr;       // nnz
   // Compressed fibres (the CSF of mode order [n, others ascending] cut above its
-  // leaf level): fibre f holds the sorted nonzeros fstart[f] .. fstart[f+1] that
-  // share the prefix (i_n, i_o1, .., i_o(D-1)), D = nd - 1; fpre is D x nfib (SoA:
-  // row, then the prefix's other indices).  The leaves are the last other mode's
-  // indices (oth + (D-1) * nnz) and vals.  chunk_fib[c] is the fibre holding
-  // nonzero c * SP_CSF_W.
+  // leaf level): fibre f is the f-th run of sorted nonzeros sharing the prefix
+  // (i_n, i_o1, .., i_o(D-1)), D = nd - 1; fpre is D x nfib (SoA: row, then the
+  // prefix's other indices).  The leaves are leafe (nnz words: the last other
+  // mode's index, bit 31 set on the last nonzero of each fibre) and vals.
+  // chunk_fib[c] is the fibre holding nonzero c * SP_CSF_W.
   i64 nfib = 0;
-  int32_t* fstart = nullptr;   // nfib + 1
+  int32_t* leafe = nullptr;    // nnz
   int32_t* fpre = nullptr;     // D x nfib
   int32_t* chunk_fib = nullptr;
   i64 nchunks = 0;
