@@ -451,7 +451,7 @@ __global__ __launch_bounds__(256) void sp_mttkrp_kernel(SpArgs a, float* __restr
 // every factor gather issued before any is used.  Same carry contract as
 // sp_mttkrp_kernel: the worker's first row goes to carry slot 2w, its last row
 // to slot 2w + 1, rows strictly inside the run are stored to `out` directly.
-constexpr int SP_W = 512;    // nonzeros per vector worker
+constexpr int SP_W = 128;    // nonzeros per vector worker
 constexpr int SP_SUB = 32;   // nonzeros staged per pass
 constexpr int SP_VU = 8;     // nonzeros in flight per worker
 
@@ -660,29 +660,48 @@ __global__ __launch_bounds__(256) void sp_mttkrp_csf_kernel(SpCsfArgs a, float* 
         stage_window();
         __builtin_amdgcn_wave_barrier();
       }
-      floatx4 fl[SP_VU];
+      int le[SP_VU];
       float v[SP_VU];
+#pragma unroll
+      for (int u = 0; u < SP_VU; u += 4) {
+        // staged entries past n are zero: leaf 0, no fibre end, value 0
+        const intx4 l4 = *reinterpret_cast<const intx4*>(sLeaf + i0 + u);
+        const intx4 v4 = *reinterpret_cast<const intx4*>(sVal + i0 + u);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          le[u + k] = l4[k];
+          v[u + k] = __int_as_float(v4[k]);
+        }
+      }
       int fw[SP_VU];     // window index of the nonzero's fibre
       bool done[SP_VU];  // the nonzero closes its fibre's contribution
       int ends = 0;
 #pragma unroll
       for (int u = 0; u < SP_VU; ++u) {
-        const int32_t le = sLeaf[i0 + u];  // staged entries past n are zero: leaf 0, no end
-        const bool in = i0 + u < n;
         fw[u] = (int)(fb - wlo) + ends;
-        done[u] = in && (le < 0 || base + i0 + u == e1 - 1);
-        ends += le < 0 ? 1 : 0;
-        v[u] = __int_as_float(sVal[i0 + u]);
-        fl[u] = *reinterpret_cast<const floatx4*>(a.F[D - 1] + col + (i64)(le & 0x7fffffff) * a.R);
+        done[u] = i0 + u < n && (le[u] < 0 || base + i0 + u == e1 - 1);
+        ends += le[u] < 0 ? 1 : 0;
       }
+      // the fibres' rows and middle indices, all LDS reads issued together
+      int rr[SP_VU];
+      int mi[D > 1 ? D - 1 : 1][SP_VU];
+#pragma unroll
+      for (int u = 0; u < SP_VU; ++u) {
+        rr[u] = sPre[fw[u]];
+#pragma unroll
+        for (int l = 1; l < D; ++l) mi[l - 1][u] = sPre[l * SP_FWIN + fw[u]];
+      }
+      floatx4 fl[SP_VU];
+#pragma unroll
+      for (int u = 0; u < SP_VU; ++u)
+        fl[u] = *reinterpret_cast<const floatx4*>(a.F[D - 1] + col + (i64)(le[u] & 0x7fffffff) * a.R);
       floatx4 mid[D > 1 ? D - 1 : 1][SP_VU];
       if constexpr (D > 1) {
 #pragma unroll
         for (int l = 1; l < D; ++l)
 #pragma unroll
           for (int u = 0; u < SP_VU; ++u)
-            mid[l - 1][u] = done[u] ? *reinterpret_cast<const floatx4*>(a.F[l - 1] + col +
-                                                                        (i64)sPre[l * SP_FWIN + fw[u]] * a.R)
+            mid[l - 1][u] = done[u] ? *reinterpret_cast<const floatx4*>(a.F[l - 1] + col + (i64)mi[l - 1][u] * a.R)
                                     : floatx4{0.f, 0.f, 0.f, 0.f};
       }
 #pragma unroll
@@ -695,10 +714,9 @@ __global__ __launch_bounds__(256) void sp_mttkrp_csf_kernel(SpCsfArgs a, float* 
 #pragma unroll
             for (int l = 0; l < D - 1; ++l) t *= mid[l][u];
           }
-          const int r = sPre[fw[u]];
-          if (r != cur) {
+          if (rr[u] != cur) {
             flush(cur, acc);
-            cur = r;
+            cur = rr[u];
             acc = floatx4{0.f, 0.f, 0.f, 0.f};
           }
           acc += t;
@@ -732,6 +750,7 @@ __global__ __launch_bounds__(256) void sp_mttkrp_csf_kernel(SpCsfArgs a, float* 
 // tile).  With `final` set a single worker stores every key.  Fixed order, so
 // the result is deterministic.
 constexpr int SP_TILE = 32;
+constexpr int SP_SEG = 8;  // slots loaded together
 
 template <int LPN>
 __global__ __launch_bounds__(256) void sp_segreduce_kernel(const float* __restrict__ ival, const int32_t* __restrict__ ikey,
@@ -750,18 +769,30 @@ __global__ __launch_bounds__(256) void sp_segreduce_kernel(const float* __restri
   const int first = ikey[k0];
   int cur = first;
   double acc = 0.0;
-  for (i64 k = k0; k < k1; ++k) {
-    const int key = ikey[k];
-    if (key != cur) {
-      if (cur == first && !final_pass) {
-        if (live) oval[(2 * w) * R + col] = (float)acc;
-      } else if (live) {
-        out[(i64)cur * R + col] = (float)acc;
-      }
-      cur = key;
-      acc = 0.0;
+  // SP_SEG slots per round: their keys and values are loaded before any is used
+  for (i64 kb = k0; kb < k1; kb += SP_SEG) {
+    int kk[SP_SEG];
+    float vv[SP_SEG];
+#pragma unroll
+    for (int u = 0; u < SP_SEG; ++u) {
+      const bool ok = kb + u < k1;
+      kk[u] = ok ? ikey[kb + u] : cur;
+      vv[u] = ok ? ival[(kb + u) * R + col] : 0.f;
     }
-    acc += (double)ival[k * R + col];
+#pragma unroll
+    for (int u = 0; u < SP_SEG; ++u) {
+      if (kb + u >= k1) break;
+      if (kk[u] != cur) {
+        if (cur == first && !final_pass) {
+          if (live) oval[(2 * w) * R + col] = (float)acc;
+        } else if (live) {
+          out[(i64)cur * R + col] = (float)acc;
+        }
+        cur = kk[u];
+        acc = 0.0;
+      }
+      acc += (double)vv[u];
+    }
   }
   if (final_pass) {
     if (live) out[(i64)cur * R + col] = (float)acc;

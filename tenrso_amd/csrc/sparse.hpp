@@ -27,7 +27,7 @@ struct SpMode {
   i64 nchunks = 0;
 };
 
-constexpr int SP_CSF_W = 512;  // nonzeros per worker of the fibre kernel
+constexpr int SP_CSF_W = 128;  // nonzeros per worker of the fibre kernel
 
 }  // namespace tenrso
 
