@@ -23,6 +23,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 METRIC = "CP-ALS s/iter + MTTKRP GB/s, 256³ r=64: BF16-MFMA vs FP32, fit Δ≤1e-4"
+TIMED_CALLS = 5  # timed cp_als calls of K iterations; the median is reported
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
@@ -292,6 +293,31 @@ def pmc_probe(Xd, Fd, shape, R):
     print("pmc probe done", flush=True)
 
 
+def pmc_probe_sparse():
+    """The sparse workload for rocprofv3 --pmc runs: a 256 MiB device copy three
+    times (calibration), then three rank-32 MTTKRPs per mode of the 20M-nonzero
+    Zipf tensor."""
+    import numpy as np
+    import torch
+    from tenrso_amd import synth
+    from tenrso_amd.sparse import SparseTensor
+    src = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    idx, vals = synth.sparse_zipf()
+    shape, R = (10000, 8000, 6000), 32
+    sp = SparseTensor(idx, vals, shape)
+    rng = np.random.default_rng(3)
+    F = [torch.from_numpy(rng.standard_normal((n, R)).astype(np.float32)).cuda() for n in shape]
+    for mode in range(3):
+        for _ in range(3):
+            sp.mttkrp(F, mode)
+    torch.cuda.synchronize()
+    print("pmc sparse probe done", flush=True)
+    sp.close()
+
+
 def read_traffic():
     """HBM bytes per launch measured by the committed rocprofv3 --pmc summary (a line
     'TRAFFIC_JSON {...}'), or None."""
@@ -316,6 +342,9 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="no timing: launch a 256 MiB device copy (counter calibration) and each MTTKRP kernel a few "
                          "times, for rocprofv3 --pmc runs")
+    ap.add_argument("--pmc-probe-sparse", action="store_true",
+                    help="no timing: a 256 MiB device copy and three MTTKRPs per mode of the 20M-nonzero sparse "
+                         "tensor, for rocprofv3 --pmc runs")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-mode MTTKRP kernel timings (--reps launches each), for rocprofv3 --stats runs")
     args = ap.parse_args()
@@ -338,18 +367,30 @@ def main():
     if args.pmc_probe:
         pmc_probe(Xd, Fd, shape, R)
         return
+    if args.pmc_probe_sparse:
+        pmc_probe_sparse()
+        return
 
-    def timed_als(precision):
+    def timed_als(precision, calls=TIMED_CALLS):
+        """W untimed warmup iterations, then `calls` timed cp_als calls of exactly K
+        iterations each, every one bracketed by synchronisations and HIP events on
+        the launch stream; the median call gives ms per iteration (the first call
+        after an idle host gap runs on a GPU that is still raising its clocks)."""
         decomp.cp_als_with_init(Xd, Fd, max(W, 1), 0.0, precision=precision)  # warmup (W iterations)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        e0.record()
-        cp = decomp.cp_als_with_init(Xd, Fd, K, 0.0, precision=precision)
-        e1.record()
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        return cp, e0.elapsed_time(e1) / K, wall
+        per, walls, cp = [], [], None
+        for _ in range(calls):
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            e0.record()
+            cp = decomp.cp_als_with_init(Xd, Fd, K, 0.0, precision=precision)
+            e1.record()
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
+            per.append(e0.elapsed_time(e1) / K)
+        order = sorted(range(calls), key=lambda i: per[i])
+        mid = order[calls // 2]
+        return cp, per[mid], walls[mid], per
 
     def kernel_timings():
         # per-mode MTTKRP kernels, timed with HIP events on the launch stream
@@ -372,8 +413,8 @@ def main():
         print(json.dumps({"mttkrp_per_mode": kernel_timings(), "reps": args.reps}), flush=True)
         return
 
-    cp32, ms32, wall32 = timed_als("fp32")
-    cp16, ms16, wall16 = timed_als("bf16")
+    cp32, ms32, wall32, calls32 = timed_als("fp32")
+    cp16, ms16, wall16, calls16 = timed_als("bf16")
     modes = kernel_timings()
     # dominant kernel: the fp32 MFMA instance of modes 0 and 1 (q-contiguous tiles)
     t_dom = (modes["0"]["fp32_us"] + modes["1"]["fp32_us"]) / 2.0
@@ -411,12 +452,13 @@ def main():
         "data": "synthetic (seeded CP model tensor, numpy PCG64)",
         "config": {"workload": "dense_256_r64: 256x256x256 f32, rank-64 CP model + noise 1e-3, cp_als rank 64, "
                                "seeded N(0,1) init, tol 0 (fixed iteration count)",
-                   "shape": list(shape), "rank": R, "iters_timed": K},
+                   "shape": list(shape), "rank": R, "iters_timed": K,
+                   "timing": f"median of {TIMED_CALLS} timed cp_als calls of K iterations each (HIP events)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "bf16": {"s_per_iter": ms16 / 1e3, "ms_per_step": ms16, "speedup_vs_fp32": ms32 / ms16,
-                 "roofline": roof_bf16, "fit": cp16.fit},
-        "fp32": {"s_per_iter": ms32 / 1e3, "fit": cp32.fit, "wall_s": wall32},
+                 "roofline": roof_bf16, "fit": cp16.fit, "ms_per_step_each_call": calls16},
+        "fp32": {"s_per_iter": ms32 / 1e3, "fit": cp32.fit, "wall_s": wall32, "ms_per_step_each_call": calls32},
         "mttkrp_per_mode": modes,
         # the >= 50x claim: ALS s/iter against the faithful single-thread port; the
         # MTTKRP line compares the FP32 kernel with the all-cores OpenMP restatement

@@ -24,45 +24,53 @@ namespace tenrso {
 
 static inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); }
 
-// One workgroup per row i of mode n (256 threads = 4 quarters x 64 columns):
-//   M[i, c]    = FP64 sum of the MTTKRP split partials of (i, c), each quarter of
-//                the splits summed by one thread group, quarters added in order;
+// One workgroup of 1024 threads per row i of mode n; thread t is column
+// c = t % CW of thread group t / CW (CW = 32 for R <= 32, else 64; NG = 1024 / CW
+// groups):
+//   M[i, c]    = FP64 sum of the MTTKRP split partials of (i, c): group g sums a
+//                contiguous range of the splits (four independent chains, all
+//                loads in flight), the NG group sums added in group order;
 //   grad[i, c] = sum_r A[i, r] Gamma_n[r, c] - M[i, c], Gamma_n = hadamard_{k != n}
-//                G_k formed on the fly (each thread its quarter of column c, loaded
-//                beside the partials), r split over the same quarters;
+//                G_k formed on the fly: group g forms the terms of its range of
+//                r (the Gram loads issued beside the partials), the group sums
+//                added in order;
 //   ip_part[i] = sum_c M[i, c] A[i, c]  (the loss's <X, Xhat> for the last mode).
 // grad and ip_part may be null.  This replaces a reduction launch, a Gamma launch
-// and a dot-product launch per mode; every sum has a fixed order.
+// and a dot-product launch per mode; every sum has a fixed order.  (A first form
+// of 256 threads in four split quarters ran each quarter's ~40 loads in chains
+// of four: 15 us per call at 96^4 r32 with 170 splits.)
 constexpr int GR_NE = 64;  // rank bound (SOLVE_RMAX)
-__global__ __launch_bounds__(256) void grad_rows_partials_kernel(const float* __restrict__ part, int splits,
-                                                                 i64 rows, int R, const double* __restrict__ G,
-                                                                 int nd, int n, const float* __restrict__ A,
-                                                                 float* __restrict__ grad,
-                                                                 double* __restrict__ ip_part) {
-  __shared__ double sq[4][GR_NE];
+constexpr int GR_T = 1024;
+__global__ __launch_bounds__(GR_T) void grad_rows_partials_kernel(const float* __restrict__ part, int splits,
+                                                                  i64 rows, int R, const double* __restrict__ G,
+                                                                  int nd, int n, const float* __restrict__ A,
+                                                                  float* __restrict__ grad,
+                                                                  double* __restrict__ ip_part) {
+  __shared__ double sq[GR_T];
   __shared__ double sM[GR_NE];
   __shared__ double sA[GR_NE];
-  const int c = threadIdx.x & 63, qd = threadIdx.x >> 6;
+  const int cw = R <= 32 ? 32 : 64, ng = GR_T / cw;
+  const int t = threadIdx.x, c = t % cw, grp = t / cw;
   const i64 row = blockIdx.x;
   const bool colok = c < R;
-  const int r0 = qd * R / 4, r1 = (qd + 1) * R / 4;
-  double w[GR_NE / 4];
-  if (grad) {
+  // this group's rows r of Gamma_n's column c (at most four: R / ng <= 4)
+  const int r0 = grp * R / ng, r1 = (grp + 1) * R / ng;
+  double w[4] = {0.0, 0.0, 0.0, 0.0};
+  if (grad && colok) {
 #pragma unroll
-    for (int k = 0; k < GR_NE / 4; ++k) {
-      double v = 0.0;
-      if (colok && r0 + k < r1) {
-        v = 1.0;
+    for (int k = 0; k < 4; ++k) {
+      if (r0 + k < r1) {
+        double v = 1.0;
         for (int m = 0; m < nd; ++m)
           if (m != n) v = v * G[((i64)m * R + r0 + k) * R + c];
+        w[k] = v;
       }
-      w[k] = v;
     }
   }
-  double m = 0.0;
+  double msum = 0.0;
   if (colok) {
     const i64 nn = rows * R, off = row * R + c;
-    const int s0 = qd * splits / 4, s1 = (qd + 1) * splits / 4;
+    const int s0 = (int)((i64)grp * splits / ng), s1 = (int)((i64)(grp + 1) * splits / ng);
     double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
     int sp = s0;
     for (; sp + 4 <= s1; sp += 4) {
@@ -72,41 +80,54 @@ __global__ __launch_bounds__(256) void grad_rows_partials_kernel(const float* __
       a3 += (double)part[(i64)(sp + 3) * nn + off];
     }
     for (; sp < s1; ++sp) a0 += (double)part[(i64)sp * nn + off];
-    m = (a0 + a1) + (a2 + a3);
-    if (qd == 0) sA[c] = (double)A[row * R + c];
+    msum = (a0 + a1) + (a2 + a3);
   }
-  sq[qd][c] = m;
+  if (t < R) sA[t] = (double)A[row * R + t];
+  sq[t] = msum;
   __syncthreads();
-  if (qd == 0) sM[c] = (sq[0][c] + sq[1][c]) + (sq[2][c] + sq[3][c]);
+  if (t < R) {
+    double s = 0.0;
+    for (int g = 0; g < ng; ++g) s += sq[g * cw + t];
+    sM[t] = s;
+  }
   __syncthreads();
   if (grad) {
-    double x0 = 0.0, x1 = 0.0;
+    double x = 0.0;
     if (colok) {
 #pragma unroll
-      for (int k = 0; k < GR_NE / 4; k += 2) {
-        if (r0 + k < r1) x0 += sA[r0 + k] * w[k];
-        if (r0 + k + 1 < r1) x1 += sA[r0 + k + 1] * w[k + 1];
-      }
+      for (int k = 0; k < 4; ++k)
+        if (r0 + k < r1) x += sA[r0 + k] * w[k];
     }
-    sq[qd][c] = x0 + x1;
+    sq[t] = x;
     __syncthreads();
-    if (qd == 0 && colok)
-      grad[row * R + c] = (float)(((sq[0][c] + sq[1][c]) + (sq[2][c] + sq[3][c])) - sM[c]);
+    if (t < R) {
+      double s = 0.0;
+      for (int g = 0; g < ng; ++g) s += sq[g * cw + t];
+      grad[row * R + t] = (float)(s - sM[t]);
+    }
   }
-  if (ip_part && qd == 0) {
-    double ip = colok ? sM[c] * sA[c] : 0.0;
+  if (ip_part && t < 64) {
+    double ip = t < R ? sM[t] * sA[t] : 0.0;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ip += __shfl_xor(ip, o, 64);
-    if (c == 0) ip_part[row] = ip;
+    if (t == 0) ip_part[row] = ip;
   }
 }
 
 // loss = 0.5 (||X||^2 - 2 sum(part) + sum_rs prod_k G_k[r,s]) into out[0] and, when
 // hist is not null, into *hist (the device-side loss history of a descent)
+__device__ void gamma_norm_block(const double* __restrict__ G, int nd, int R, int n, double* __restrict__ out);
+
+// Workgroup 0: the loss.  With gnorm set, workgroups 1 .. nd: ||Gamma_n||_F of
+// mode n = blockIdx.x - 1 into gnorm[n] (the step size of a descent step).
 __global__ __launch_bounds__(256) void loss_kernel(const double* __restrict__ G, int nd, int R,
                                                    const double* __restrict__ part, int npart,
                                                    const double* __restrict__ scal, double* __restrict__ out,
-                                                   double* __restrict__ hist) {
+                                                   double* __restrict__ hist, double* __restrict__ gnorm) {
+  if (blockIdx.x > 0) {
+    gamma_norm_block(G, nd, R, (int)blockIdx.x - 1, gnorm);
+    return;
+  }
   __shared__ double red[256];
   double rec = 0.0;
   for (int e = threadIdx.x; e < R * R; e += 256) {
@@ -140,26 +161,23 @@ __global__ __launch_bounds__(256) void loss_kernel(const double* __restrict__ G,
 }
 
 // ||Gamma_n||_F for every mode (used by the automatic step size)
-__global__ __launch_bounds__(256) void gamma_norms_kernel(const double* __restrict__ G, int nd, int R,
-                                                          double* __restrict__ out) {
+// ||Gamma_n||_F of mode n into out[n], by one workgroup of 256 threads
+__device__ void gamma_norm_block(const double* __restrict__ G, int nd, int R, int n, double* __restrict__ out) {
   __shared__ double red[256];
-  for (int n = 0; n < nd; ++n) {
-    double s = 0.0;
-    for (int e = threadIdx.x; e < R * R; e += 256) {
-      double v = 1.0;
-      for (int k = 0; k < nd; ++k)
-        if (k != n) v = v * G[(i64)k * R * R + e];
-      s += v * v;
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-      if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[n] = sqrt(red[0]);
+  double s = 0.0;
+  for (int e = threadIdx.x; e < R * R; e += 256) {
+    double v = 1.0;
+    for (int k = 0; k < nd; ++k)
+      if (k != n) v = v * G[(i64)k * R * R + e];
+    s += v * v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[n] = sqrt(red[0]);
 }
 
 // A_k -= step * grad_k for every factor in one launch (the factors' elements are
@@ -257,22 +275,22 @@ int loss_work(tenrso_ctx* ctx, int nd, const i64* shape, i64 R, LossWork* w) {
 // MTTKRP as split partials, then one fused row kernel; modes without a gradient
 // to form are skipped except the last, whose MTTKRP gives <X, Xhat>.
 static int loss_grad_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, const float* const* F, i64 R,
-                            float* const* grads, LossWork& w, double* hist) {
+                            float* const* grads, LossWork& w, double* hist, const float* Xt = nullptr,
+                            double* gnorm = nullptr) {
   TENRSO_TRY(gram_f64_many(ctx, nd, F, shape, R, w.G));
   for (int n = 0; n < nd; ++n) {
     float* gn = (grads && grads[n]) ? grads[n] : nullptr;
     const bool last = n == nd - 1;
     if (!gn && !last) continue;
     MttkrpResult mr;
-    TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, X, nullptr, nd, shape, F, R, n, MttkrpOut::PARTIAL, nullptr,
-                            &mr));
+    TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, X, Xt, nd, shape, F, R, n, MttkrpOut::PARTIAL, nullptr, &mr));
     if (shape[n] == 0) continue;
-    hipLaunchKernelGGL(grad_rows_partials_kernel, dim3((unsigned)shape[n]), dim3(256), 0, ctx->stream, mr.partial,
+    hipLaunchKernelGGL(grad_rows_partials_kernel, dim3((unsigned)shape[n]), dim3(GR_T), 0, ctx->stream, mr.partial,
                        mr.splits, shape[n], (int)R, w.G, nd, n, F[n], gn, last ? w.part : nullptr);
     TENRSO_HIP_TRY(hipGetLastError());
   }
-  hipLaunchKernelGGL(loss_kernel, dim3(1), dim3(256), 0, ctx->stream, w.G, nd, (int)R, w.part, (int)shape[nd - 1],
-                     w.scal, w.out, hist);
+  hipLaunchKernelGGL(loss_kernel, dim3(gnorm ? 1 + nd : 1), dim3(256), 0, ctx->stream, w.G, nd, (int)R, w.part,
+                     (int)shape[nd - 1], w.scal, w.out, hist, gnorm);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -337,12 +355,21 @@ int cp_gd(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, float* cons
   am.off[nd] = off;
   TENRSO_TRY(fill_f64(ctx, w.scal, 64, 0.0));
   TENRSO_TRY(tensor_norm_min(ctx, X, total, w.part, w.scal));
+  // X does not change over the steps: one last-to-front copy lets the last
+  // mode's MTTKRP of every step stream X with 16-byte q-contiguous loads
+  // (measured at 96^4 r32: that mode from 82 to the ~63 us of mode 0)
+  const float* Xt = nullptr;
+  if (steps > 1 && nd >= 3 && total > 0) {
+    void* pt = nullptr;
+    TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * sizeof(float), &pt));
+    TENRSO_TRY(transpose_last_to_front_f32(ctx, X, nd, shape, (float*)pt));
+    Xt = (const float*)pt;
+  }
   for (int s = 0; s < steps; ++s) {
     // loss at the point the step is taken from
+    // with lr <= 0 the loss launch also leaves ||Gamma_n||_F (the step size)
     TENRSO_TRY(loss_grad_device(ctx, X, nd, shape, (const float* const*)F, R, grads.data(), w,
-                                hist_dev ? hist_dev + s : nullptr));
-    if (lr <= 0.0)
-      hipLaunchKernelGGL(gamma_norms_kernel, dim3(1), dim3(256), 0, ctx->stream, w.G, nd, (int)R, w.out + 3);
+                                hist_dev ? hist_dev + s : nullptr, Xt, lr <= 0.0 ? w.out + 3 : nullptr));
     if (ftot > 0)
       hipLaunchKernelGGL(axpy_step_kernel, dim3(cdiv(ftot, 256)), dim3(256), 0, ctx->stream, am, nd, lr, w.out + 3);
     TENRSO_HIP_TRY(hipGetLastError());

@@ -482,6 +482,18 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
 #pragma unroll
           for (int h = 0; h < 2; ++h)
             b.x[mt][h] = *reinterpret_cast<const floatx4*>(((okm >> (4 * h)) & 1u) ? base + 16 * h : X);
+        } else if constexpr (XL == 3) {
+          // an even fastest mode (and row length) on an 8-byte aligned X: pairs,
+          // all in or all out
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int pr = 0; pr < 2; ++pr) {
+              const float2 v =
+                  *reinterpret_cast<const float2*>(((okm >> (4 * h + 2 * pr)) & 1u) ? base + 16 * h + 2 * pr : X);
+              b.x[mt][h][2 * pr] = v.x;
+              b.x[mt][h][2 * pr + 1] = v.y;
+            }
         } else {
 #pragma unroll
           for (int h = 0; h < 2; ++h)
@@ -1227,10 +1239,12 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
   for (int k = 0; k < nd; ++k) shape_use[k] = shape[k];
 
   // BF16 last mode: run mode 0 of the last-to-front copy (same j order); an
-  // fp32 copy is contracted exactly (bf16x3 split)
+  // fp32 copy is contracted exactly (bf16x3 split).  FP32 with a last-to-front
+  // fp32 copy given: its last mode also runs as mode 0 of the copy (q-contiguous
+  // 16-byte loads instead of the last mode's i-contiguous ones).
   bool x3 = false;
-  if (bf16 && mode == nd - 1 && nd >= 2) {
-    x3 = xt_f32;
+  if ((bf16 || X_t) && mode == nd - 1 && nd >= 2) {
+    x3 = bf16 && xt_f32;
     if (!X_t) {
       set_error("internal: bf16 last-mode MTTKRP needs the last-to-front copy");
       return TENRSO_ERR_INVALID_ARGUMENT;
@@ -1350,8 +1364,12 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
   // bf16 kernels need 16-byte alignment, else the generic kernel runs.
   const bool aligned16 = (uintptr_t)Xuse % 16 == 0;
   if (!bf16 && !caseB && !aligned16 && kw > 0) XL = 2;
+  // element-wise fp32 loads become pairs (8 bytes) where every row start is even
+  // and X is 8-byte aligned (a fastest mode of 250, say)
+  if (XL == 2 && fast_stride % 2 == 0 && Qmem % 2 == 0 && (uintptr_t)Xuse % 8 == 0) XL = 3;
   const bool mfma_ok = pl.n_oth >= 1 && pl.J > 0 && kw > 0 && !(bf16 && caseB) && I > 0 &&
-                       (uintptr_t)Xuse % 4 == 0 && (bf16 ? aligned16 && XL != 2 : (caseB || XL == 2 || aligned16));
+                       (uintptr_t)Xuse % 4 == 0 &&
+                       (bf16 ? aligned16 && XL < 2 : (caseB || XL >= 2 || aligned16));
 
   float* partial = nullptr;
   int splits = 1;
@@ -1376,8 +1394,8 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     auto with_kernel = [&](auto&& f) {
       auto pick = [&](auto wv, auto xl) {
         constexpr bool WV = decltype(wv)::value;
-        constexpr int L = decltype(xl)::value;  // XL; 2 only for fp32 case A
-        constexpr int LB = L == 2 ? 1 : L;      // instances that have no element-wise form
+        constexpr int L = decltype(xl)::value;  // XL; 2 and 3 only for fp32 case A
+        constexpr int LB = L >= 2 ? 1 : L;      // instances that have no element- or pair-wise form
         if (!bf16) {
           if (!caseB) {
             if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, false, KWMAX, WV, L>);
@@ -1410,6 +1428,7 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
       auto pick_xl = [&](auto wv) {
         if (XL == 0) return pick(wv, std::integral_constant<int, 0>{});
         if (XL == 1) return pick(wv, std::integral_constant<int, 1>{});
+        if (XL == 3) return pick(wv, std::integral_constant<int, 3>{});
         return pick(wv, std::integral_constant<int, 2>{});
       };
       return a.win_vec ? pick_xl(std::true_type{}) : pick_xl(std::false_type{});
