@@ -278,12 +278,20 @@ static int loss_grad_device(tenrso_ctx* ctx, const float* X, int nd, const i64* 
                             float* const* grads, LossWork& w, double* hist, const float* Xt = nullptr,
                             double* gnorm = nullptr) {
   TENRSO_TRY(gram_f64_many(ctx, nd, F, shape, R, w.G));
+  // tensors of order >= 4: every mode's Khatri-Rao PRE rows in one launch
+  const float* pre[MAXD] = {};
+  if (nd >= 4) {
+    int modes[MAXD];
+    for (int n = 0; n < nd; ++n) modes[n] = n;
+    TENRSO_TRY(mttkrp_pre_batch(ctx, nd, shape, F, R, nd, modes, Xt != nullptr, pre));
+  }
   for (int n = 0; n < nd; ++n) {
     float* gn = (grads && grads[n]) ? grads[n] : nullptr;
     const bool last = n == nd - 1;
     if (!gn && !last) continue;
     MttkrpResult mr;
-    TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, X, Xt, nd, shape, F, R, n, MttkrpOut::PARTIAL, nullptr, &mr));
+    TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, X, Xt, nd, shape, F, R, n, MttkrpOut::PARTIAL, nullptr, &mr,
+                            nullptr, false, nullptr, 0, 0, pre[n]));
     if (shape[n] == 0) continue;
     hipLaunchKernelGGL(grad_rows_partials_kernel, dim3((unsigned)shape[n]), dim3(GR_T), 0, ctx->stream, mr.partial,
                        mr.splits, shape[n], (int)R, w.G, nd, n, F[n], gn, last ? w.part : nullptr);

@@ -78,7 +78,7 @@ enum BufSlot : int {
   SLOT_PARTIAL = 0,
   SLOT_M64,
   SLOT_GRAMS,
-  SLOT_GRAM_PART,
+  SLOT_PRE,
   SLOT_SOLVE,
   SLOT_SCALARS,
   SLOT_XBF16,

@@ -37,10 +37,20 @@ struct MttkrpScratch {
 // MTTKRP launch (or, where it cannot, as its own kernel just before it).
 // x_pad / xt_pad (BF16 only, 0 = none): the memory extent of the last mode of X
 // and of X_t when those copies have it padded with zeros.
+// pre_rows (optional, tensors of order >= 4): the PRE Khatri-Rao rows of this
+// mode already formed (mttkrp_pre_batch), instead of a launch that forms them.
 int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int nd, const i64* shape,
                  const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res,
                  const AlsJob* job = nullptr, bool xt_f32 = false, const MttkrpScratch* scratch = nullptr,
-                 i64 x_pad = 0, i64 xt_pad = 0);
+                 i64 x_pad = 0, i64 xt_pad = 0, const float* pre_rows = nullptr);
+
+// The PRE Khatri-Rao rows of the MTTKRPs of several modes of one tensor (order
+// >= 4) in one launch: pre[i] receives the rows the MFMA MTTKRP of modes[i]
+// contracts (via_copy_last: the last mode runs as mode 0 of the last-to-front
+// copy), null where the mode needs none; modes with the same rows share them.
+// The rows live in the context's SLOT_PRE buffer until the next call.
+int mttkrp_pre_batch(tenrso_ctx* ctx, int nd, const i64* shape, const float* const* F, i64 R, int nmodes,
+                     const int* modes, bool via_copy_last, const float** pre);
 
 // Size of the file-scope workspace of the no-allocation MTTKRP entry point
 // (noalloc.hip): 16 MiB per GPU.

@@ -968,7 +968,7 @@ struct KrManyArgs {
   i64 sizes[MAXD];
   const float* F[MAXD];
 };
-__global__ __launch_bounds__(256) void kr_many_kernel(KrManyArgs a, float* __restrict__ dst) {
+__device__ __forceinline__ void kr_many_body(const KrManyArgs& a, float* __restrict__ dst) {
   const i64 idx = (i64)blockIdx.x * 256 + threadIdx.x;
   if (idx >= a.rows * a.R) return;
   const i64 row = idx / a.R, r = idx - (idx / a.R) * a.R;
@@ -983,6 +983,7 @@ __global__ __launch_bounds__(256) void kr_many_kernel(KrManyArgs a, float* __res
   for (int m = 0; m < a.n; ++m) v = v * vals[m];  // left-to-right, as khatri_rao_except_mode
   dst[idx] = v;
 }
+__global__ __launch_bounds__(256) void kr_many_kernel(KrManyArgs a, float* __restrict__ dst) { kr_many_body(a, dst); }
 
 // ---------------------------------------------------------------------------
 // host side
@@ -993,6 +994,72 @@ int reduce_partials(tenrso_ctx* ctx, const float* partial, int splits, i64 n, fl
   if (n == 0) return TENRSO_OK;
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(cdiv(n, 64)), dim3(64 * RED_GROUPS), 0, ctx->stream, partial,
                      splits, n, out32, out64);
+  TENRSO_HIP_TRY(hipGetLastError());
+  return TENRSO_OK;
+}
+
+// Several Khatri-Rao products in one launch: product y of grid.y into dst[y].
+constexpr int KR_BATCH = MAXD;
+struct KrBatchArgs {
+  int count;
+  KrManyArgs k[KR_BATCH];
+  float* dst[KR_BATCH];
+};
+__global__ __launch_bounds__(256) void kr_batch_kernel(KrBatchArgs b) {
+  const int y = blockIdx.y;
+  if (y >= b.count) return;
+  kr_many_body(b.k[y], b.dst[y]);
+}
+
+int mttkrp_pre_batch(tenrso_ctx* ctx, int nd, const i64* shape, const float* const* F, i64 R, int nmodes,
+                     const int* modes, bool via_copy_last, const float** pre) {
+  KrBatchArgs b{};
+  int owner[KR_BATCH];  // the product each requested mode uses, or -1
+  i64 offs[KR_BATCH + 1] = {0};
+  for (int i = 0; i < nmodes; ++i) {
+    pre[i] = nullptr;
+    owner[i] = -1;
+    const int mode = modes[i];
+    // the PRE factors: every other mode but the fastest, in order (the last
+    // mode through the copy contracts modes 0 .. nd-3 the same way)
+    KrManyArgs k{};
+    k.R = R;
+    k.rows = 1;
+    const bool copy = via_copy_last && mode == nd - 1;
+    const int n_oth = nd - 1;
+    if (n_oth < 3) continue;
+    for (int j = 0; j < nd && k.n < n_oth - 1; ++j) {
+      if (!copy && j == mode) continue;
+      k.F[k.n] = F[j];
+      k.sizes[k.n] = shape[j];
+      k.rows *= shape[j];
+      ++k.n;
+    }
+    // shared with an earlier product of the same factors?
+    for (int q = 0; q < b.count && owner[i] < 0; ++q) {
+      bool same = b.k[q].n == k.n;
+      for (int j = 0; same && j < k.n; ++j) same = b.k[q].F[j] == k.F[j] && b.k[q].sizes[j] == k.sizes[j];
+      if (same) owner[i] = q;
+    }
+    if (owner[i] < 0) {
+      if (b.count == KR_BATCH) continue;  // more distinct products than one launch takes: the MTTKRP forms its own
+      owner[i] = b.count;
+      b.k[b.count] = k;
+      offs[b.count + 1] = offs[b.count] + k.rows * R;
+      ++b.count;
+    }
+  }
+  if (b.count == 0) return TENRSO_OK;
+  void* buf = nullptr;
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_PRE, (size_t)offs[b.count] * sizeof(float), &buf));
+  i64 maxn = 0;
+  for (int q = 0; q < b.count; ++q) {
+    b.dst[q] = (float*)buf + offs[q];
+    maxn = std::max(maxn, b.k[q].rows * R);
+  }
+  for (int i = 0; i < nmodes; ++i)
+    if (owner[i] >= 0) pre[i] = b.dst[owner[i]];
+  hipLaunchKernelGGL(kr_batch_kernel, dim3(cdiv(maxn, 256), b.count), dim3(256), 0, ctx->stream, b);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -1228,7 +1295,8 @@ Plan make_plan(int nd, const i64* shape, int mode) {
 
 int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int nd, const i64* shape,
                  const float* const* F, i64 R, int mode, MttkrpOut outk, void* out, MttkrpResult* res,
-                 const AlsJob* job, bool xt_f32, const MttkrpScratch* scratch, i64 x_pad, i64 xt_pad) {
+                 const AlsJob* job, bool xt_f32, const MttkrpScratch* scratch, i64 x_pad, i64 xt_pad,
+                 const float* pre_rows) {
   Plan pl = make_plan(nd, shape, mode);
   const bool bf16 = prec == TENRSO_PREC_BF16;
   const float* Fv[MAXD];
@@ -1290,7 +1358,9 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
               scratch_used + (size_t)need_splits * (size_t)std::max<i64>(1, nout) * sizeof(float));
     return TENRSO_ERR_UNSUPPORTED;
   };
-  if (pl.n_oth >= 3) {
+  if (pl.n_oth >= 3 && pre_rows) {
+    pre = pre_rows;
+  } else if (pl.n_oth >= 3) {
     const float* pf[MAXD];
     i64 ps[MAXD];
     i64 prows = 1;
