@@ -422,12 +422,12 @@ def main():
     traffic = read_traffic() or {}
     roofline = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_FP32_TFLOPS, "traffic": traffic.get("fp32_bytes"),
-                "kernel": "mttkrp_f32_mfma<4,4,2,false,128,true> (modes 0,1)", "avg_us": t_dom,
+                "kernel": "tenrso::mttkrp_f32_mfma<4, 4, 2, false, 128, true, 0> (modes 0, 1)", "avg_us": t_dom,
                 "algorithmic_flops": mttkrp_flops(shape, R)}
     t_b16 = (modes["0"]["bf16_us"] + modes["1"]["bf16_us"]) / 2.0
     b_alg = mttkrp_bytes(shape, R, 0, 2)
     roof_bf16 = {"bound": "hbm", "achieved": b_alg / (t_b16 * 1e-6) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                 "frac": b_alg / (t_b16 * 1e-6) / 1e9 / PEAK_HBM_GBS, "kernel": "mttkrp_bf16_mfma<4,8,2,2,128,true> (modes 0,1)",
+                 "frac": b_alg / (t_b16 * 1e-6) / 1e9 / PEAK_HBM_GBS, "kernel": "tenrso::mttkrp_bf16_mfma<unsigned short, 4, 8, 2, 2, 128, true, 0> (modes 0, 1)",
                  "avg_us": t_b16, "algorithmic_bytes": b_alg, "traffic": traffic.get("bf16_bytes")}
 
     gpu_fits = {"fp32": cp32.fit, "bf16": cp16.fit}

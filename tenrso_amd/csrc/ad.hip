@@ -362,17 +362,25 @@ int cp_gd(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, float* cons
   }
   am.off[nd] = off;
   TENRSO_TRY(fill_f64(ctx, w.scal, 64, 0.0));
-  TENRSO_TRY(tensor_norm_min(ctx, X, total, w.part, w.scal));
   // X does not change over the steps: one last-to-front copy lets the last
   // mode's MTTKRP of every step stream X with 16-byte q-contiguous loads
-  // (measured at 96^4 r32: that mode from 82 to the ~63 us of mode 0)
+  // (measured at 96^4 r32: that mode from 82 to the ~63 us of mode 0).  The
+  // copy and ||X||^2 come from one pass over X where the shape allows it.
   const float* Xt = nullptr;
+  bool normed = false;
   if (steps > 1 && nd >= 3 && total > 0) {
     void* pt = nullptr;
     TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * sizeof(float), &pt));
-    TENRSO_TRY(transpose_last_to_front_f32(ctx, X, nd, shape, (float*)pt));
+    int nblk = 0;
+    if (prepare_bf16_als(ctx, X, nd, shape, nullptr, (float*)pt, w.part, &nblk) == TENRSO_OK) {
+      TENRSO_TRY(norm_min_finish(ctx, w.part, nblk, w.scal, 0));
+      normed = true;
+    } else {
+      TENRSO_TRY(transpose_last_to_front_f32(ctx, X, nd, shape, (float*)pt));
+    }
     Xt = (const float*)pt;
   }
+  if (!normed) TENRSO_TRY(tensor_norm_min(ctx, X, total, w.part, w.scal));
   for (int s = 0; s < steps; ++s) {
     // loss at the point the step is taken from
     // with lr <= 0 the loss launch also leaves ||Gamma_n||_F (the step size)

@@ -75,7 +75,7 @@ int convert_bf16(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, bo
 // BF16 ALS input in one pass: dst = bf16(X), dstT = the fp32 tensor with its last
 // mode moved to the front, and (sum of squares, min) partials in tmp[0, 2 * nblk).
 // TENRSO_ERR_UNSUPPORTED (nothing launched) unless the last mode is a multiple of
-// 4 and X is 16-byte aligned.
+// 4 and X is 16-byte aligned.  dst may be null: then only dstT and the partials.
 int prepare_bf16_als(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, uint16_t* dst, float* dstT,
                      double* tmp, int* nblk_out);
 

@@ -1182,12 +1182,14 @@ __global__ __launch_bounds__(256) void prepare_bf16_kernel(const float* __restri
           ss = fma((double)v[q], (double)v[q], ss);
           mn = fminf(mn, v[q]);
         }
-        ushort4 o;
-        o.x = f32_to_bf16_rne(v[0]);
-        o.y = f32_to_bf16_rne(v[1]);
-        o.z = f32_to_bf16_rne(v[2]);
-        o.w = f32_to_bf16_rne(v[3]);
-        *reinterpret_cast<ushort4*>(dst + r * K + k) = o;
+        if (dst) {  // null: the transposed copy and the norm only
+          ushort4 o;
+          o.x = f32_to_bf16_rne(v[0]);
+          o.y = f32_to_bf16_rne(v[1]);
+          o.z = f32_to_bf16_rne(v[2]);
+          o.w = f32_to_bf16_rne(v[3]);
+          *reinterpret_cast<ushort4*>(dst + r * K + k) = o;
+        }
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) tile[rr][c4 + q] = v[q];
