@@ -174,6 +174,42 @@ def test_sparse_run_tail_row_changes(R):
     assert rel(got, oracle.coo_mttkrp(idx, vals, shape, F64, 0)) <= 1e-5
 
 
+def _prefix_count(idx, mode):
+    """Distinct (i_mode, i_o1 .. i_o(N-2)) prefixes: the compressed fibres of the mode."""
+    nd = idx.shape[0]
+    others = [m for m in range(nd) if m != mode]
+    keys = np.stack([idx[mode]] + [idx[m] for m in others[:-1]], axis=1)
+    return len(np.unique(keys, axis=0))
+
+
+@pytest.mark.parametrize("R", [16, 32, 64])
+def test_sparse_fibre_extremes(R):
+    """The fibre kernel at its edges: fibres of 300 nonzeros that span three
+    128-nonzero workers (modes 0 and 1), and chunks whose fibres hold one to three
+    nonzeros, more than the 64 (32) prefixes a worker's window stages at once
+    (mode 2 here, every mode of the second tensor).  Against the FP64 COO oracle,
+    and the fibre count against the distinct prefixes."""
+    print(f"[case] sparse fibre extremes R={R}", flush=True)
+    rng = np.random.default_rng(R)
+    shape = (20, 30, 500)
+    pairs = rng.choice(20 * 30, 40, replace=False)
+    ks = np.stack([rng.choice(500, 300, replace=False) for _ in pairs])
+    lin = np.ravel_multi_index((np.repeat(pairs // 30, 300), np.repeat(pairs % 30, 300), ks.reshape(-1)), shape)
+    short = (8, 2000, 3)
+    lin2 = rng.choice(int(np.prod(short)), 9000, replace=False)
+    for shp, ln in ((shape, lin), (short, lin2)):
+        idx = np.array(np.unravel_index(ln, shp), dtype=np.int32)
+        vals = rng.random(ln.size).astype(np.float32)
+        sp = SparseTensor(idx, vals, shp)
+        F, Fd = dev_factors(shp, R, seed=R + 7)
+        F64 = [f.astype(np.float64) for f in F]
+        for mode in range(3):
+            assert sp.fibres(mode) == _prefix_count(idx, mode)
+            got = sp.mttkrp(Fd, mode).cpu().numpy()
+            assert rel(got, oracle.coo_mttkrp(idx, vals, shp, F64, mode)) <= 1e-5
+        sp.close()
+
+
 def test_sparse_mttkrp_deterministic():
     idx, vals = synth.sparse_zipf(shape=(1000, 800, 600), nnz=200_000, seed=1)
     sp = SparseTensor(idx, vals, (1000, 800, 600))
