@@ -453,7 +453,7 @@ __global__ __launch_bounds__(256) void sp_mttkrp_kernel(SpArgs a, float* __restr
 // to slot 2w + 1, rows strictly inside the run are stored to `out` directly.
 constexpr int SP_W = 128;    // nonzeros per vector worker
 constexpr int SP_SUB = 32;   // nonzeros staged per pass
-constexpr int SP_VU = 8;     // nonzeros in flight per worker
+constexpr int SP_VU = 4;     // nonzeros in flight per worker (a multiple of 4: staged entries are read as 16-byte vectors)
 
 typedef int32_t intx4 __attribute__((ext_vector_type(4)));
 typedef int32_t intx2 __attribute__((ext_vector_type(2)));
