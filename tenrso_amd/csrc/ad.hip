@@ -39,13 +39,21 @@ static inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); 
 // and a dot-product launch per mode; every sum has a fixed order.  (A first form
 // of 256 threads in four split quarters ran each quarter's ~40 loads in chains
 // of four: 15 us per call at 96^4 r32 with 170 splits.)
+//
+// Tree form (TREE): M comes from a two-mode partial MTTKRP Y (split partials
+// [splits][yrows][R]) instead of the full one: M[i, c] = sum_j Y[i sa + j sb, c]
+// B[j, c], j = 0 .. J-1, each group a contiguous range of j.
 constexpr int GR_NE = 64;  // rank bound (SOLVE_RMAX)
 constexpr int GR_T = 1024;
-__global__ __launch_bounds__(GR_T) void grad_rows_partials_kernel(const float* __restrict__ part, int splits,
-                                                                  i64 rows, int R, const double* __restrict__ G,
-                                                                  int nd, int n, const float* __restrict__ A,
-                                                                  float* __restrict__ grad,
-                                                                  double* __restrict__ ip_part) {
+struct GradTree {
+  i64 yrows, J, sa, sb;
+  const float* B;
+};
+template <bool TREE>
+__global__ __launch_bounds__(GR_T) void grad_rows_kernel(const float* __restrict__ part, int splits, i64 rows,
+                                                         int R, const double* __restrict__ G, int nd, int n,
+                                                         const float* __restrict__ A, float* __restrict__ grad,
+                                                         double* __restrict__ ip_part, GradTree tr) {
   __shared__ double sq[GR_T];
   __shared__ double sM[GR_NE];
   __shared__ double sA[GR_NE];
@@ -68,7 +76,29 @@ __global__ __launch_bounds__(GR_T) void grad_rows_partials_kernel(const float* _
     }
   }
   double msum = 0.0;
-  if (colok) {
+  if (TREE && colok) {
+    const i64 nn = tr.yrows * R;
+    const i64 j0 = grp * tr.J / ng, j1 = (grp + 1) * tr.J / ng;
+    double a0 = 0, a1 = 0;
+    i64 j = j0;
+    for (; j + 2 <= j1; j += 2) {
+      const i64 y0 = (row * tr.sa + j * tr.sb) * R + c, y1 = y0 + tr.sb * R;
+      double v0 = 0.0, v1 = 0.0;
+      for (int sp = 0; sp < splits; ++sp) {
+        v0 += (double)part[(i64)sp * nn + y0];
+        v1 += (double)part[(i64)sp * nn + y1];
+      }
+      a0 += v0 * (double)tr.B[j * R + c];
+      a1 += v1 * (double)tr.B[(j + 1) * R + c];
+    }
+    if (j < j1) {
+      const i64 y0 = (row * tr.sa + j * tr.sb) * R + c;
+      double v0 = 0.0;
+      for (int sp = 0; sp < splits; ++sp) v0 += (double)part[(i64)sp * nn + y0];
+      a0 += v0 * (double)tr.B[j * R + c];
+    }
+    msum = a0 + a1;
+  } else if (colok) {
     const i64 nn = rows * R, off = row * R + c;
     const int s0 = (int)((i64)grp * splits / ng), s1 = (int)((i64)(grp + 1) * splits / ng);
     double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
@@ -276,8 +306,41 @@ int loss_work(tenrso_ctx* ctx, int nd, const i64* shape, i64 R, LossWork* w) {
 // to form are skipped except the last, whose MTTKRP gives <X, Xhat>.
 static int loss_grad_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, const float* const* F, i64 R,
                             float* const* grads, LossWork& w, double* hist, const float* Xt = nullptr,
-                            double* gnorm = nullptr) {
+                            double* gnorm = nullptr, bool tree = false) {
   TENRSO_TRY(gram_f64_many(ctx, nd, F, shape, R, w.G));
+  if (tree) {
+    // 4-way dimension tree: two passes over X instead of four.
+    //   Y01[i0 I1 + i1, r] = sum_{i2,i3} X A2[i2,r] A3[i3,r]: mode 0 of the view
+    //     [I0 I1, I2, I3];  M_0 = sum_{i1} Y01 A1,  M_1 = sum_{i0} Y01 A0;
+    //   Y23[i2 I3 + i3, r] = sum_{i0,i1} X A0[i0,r] A1[i1,r]: mode 2 (the last) of
+    //     the view [I0, I1, I2 I3];  M_2 = sum_{i3} Y23 A3,  M_3 = sum_{i2} Y23 A2.
+    // Each Y stays as the MTTKRP's split partials, consumed before the next pass
+    // reuses that buffer (stream order).
+    const i64 I0 = shape[0], I1 = shape[1], I2 = shape[2], I3 = shape[3];
+    for (int half = 0; half < 2; ++half) {
+      const i64 v[3] = {half == 0 ? I0 * I1 : I0, half == 0 ? I2 : I1, half == 0 ? I3 : I2 * I3};
+      const float* Fv[3] = {half == 0 ? F[0] : F[0], half == 0 ? F[2] : F[1], half == 0 ? F[3] : F[2]};
+      MttkrpResult mr;
+      TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, X, nullptr, 3, v, Fv, R, half == 0 ? 0 : 2,
+                              MttkrpOut::PARTIAL, nullptr, &mr));
+      const i64 ya = half == 0 ? I0 : I2, yb = half == 0 ? I1 : I3;  // Y rows = ya x yb
+      for (int k = 0; k < 2; ++k) {
+        const int n = 2 * half + k;
+        float* gn = (grads && grads[n]) ? grads[n] : nullptr;
+        const bool last = n == nd - 1;
+        if (!gn && !last) continue;
+        // mode 2 half, first: M[i] sums over the second index; second: over the first
+        GradTree tr{ya * yb, k == 0 ? yb : ya, k == 0 ? yb : 1, k == 0 ? 1 : yb, F[k == 0 ? n + 1 : n - 1]};
+        hipLaunchKernelGGL(grad_rows_kernel<true>, dim3((unsigned)shape[n]), dim3(GR_T), 0, ctx->stream, mr.partial,
+                           mr.splits, shape[n], (int)R, w.G, nd, n, F[n], gn, last ? w.part : nullptr, tr);
+        TENRSO_HIP_TRY(hipGetLastError());
+      }
+    }
+    hipLaunchKernelGGL(loss_kernel, dim3(gnorm ? 1 + nd : 1), dim3(256), 0, ctx->stream, w.G, nd, (int)R, w.part,
+                       (int)shape[nd - 1], w.scal, w.out, hist, gnorm);
+    TENRSO_HIP_TRY(hipGetLastError());
+    return TENRSO_OK;
+  }
   // tensors of order >= 4: every mode's Khatri-Rao PRE rows in one launch
   const float* pre[MAXD] = {};
   if (nd >= 4) {
@@ -293,8 +356,8 @@ static int loss_grad_device(tenrso_ctx* ctx, const float* X, int nd, const i64* 
     TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, X, Xt, nd, shape, F, R, n, MttkrpOut::PARTIAL, nullptr, &mr,
                             nullptr, false, nullptr, 0, 0, pre[n]));
     if (shape[n] == 0) continue;
-    hipLaunchKernelGGL(grad_rows_partials_kernel, dim3((unsigned)shape[n]), dim3(GR_T), 0, ctx->stream, mr.partial,
-                       mr.splits, shape[n], (int)R, w.G, nd, n, F[n], gn, last ? w.part : nullptr);
+    hipLaunchKernelGGL(grad_rows_kernel<false>, dim3((unsigned)shape[n]), dim3(GR_T), 0, ctx->stream, mr.partial,
+                       mr.splits, shape[n], (int)R, w.G, nd, n, F[n], gn, last ? w.part : nullptr, GradTree{});
     TENRSO_HIP_TRY(hipGetLastError());
   }
   hipLaunchKernelGGL(loss_kernel, dim3(gnorm ? 1 + nd : 1), dim3(256), 0, ctx->stream, w.G, nd, (int)R, w.part,
@@ -315,7 +378,7 @@ int cp_loss_grad_dev(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, 
   for (int k = 0; k < nd; ++k) total *= shape[k];
   TENRSO_TRY(fill_f64(ctx, w.scal, 64, 0.0));
   TENRSO_TRY(tensor_norm_min(ctx, X, total, w.part, w.scal));
-  TENRSO_TRY(loss_grad_device(ctx, X, nd, shape, F, R, grads, w, nullptr));
+  TENRSO_TRY(loss_grad_device(ctx, X, nd, shape, F, R, grads, w, nullptr, nullptr, nullptr, nd == 4 && total > 0));
   if (loss_dev) *loss_dev = w.out;
   return TENRSO_OK;
 }
@@ -366,9 +429,11 @@ int cp_gd(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, float* cons
   // mode's MTTKRP of every step stream X with 16-byte q-contiguous loads
   // (measured at 96^4 r32: that mode from 82 to the ~63 us of mode 0).  The
   // copy and ||X||^2 come from one pass over X where the shape allows it.
+  // Four modes take the dimension tree instead (two passes over X per step).
+  const bool tree = nd == 4 && total > 0;
   const float* Xt = nullptr;
   bool normed = false;
-  if (steps > 1 && nd >= 3 && total > 0) {
+  if (steps > 1 && nd >= 3 && total > 0 && !tree) {
     void* pt = nullptr;
     TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)total * sizeof(float), &pt));
     int nblk = 0;
@@ -385,7 +450,7 @@ int cp_gd(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, float* cons
     // loss at the point the step is taken from
     // with lr <= 0 the loss launch also leaves ||Gamma_n||_F (the step size)
     TENRSO_TRY(loss_grad_device(ctx, X, nd, shape, (const float* const*)F, R, grads.data(), w,
-                                hist_dev ? hist_dev + s : nullptr, Xt, lr <= 0.0 ? w.out + 3 : nullptr));
+                                hist_dev ? hist_dev + s : nullptr, Xt, lr <= 0.0 ? w.out + 3 : nullptr, tree));
     if (ftot > 0)
       hipLaunchKernelGGL(axpy_step_kernel, dim3(cdiv(ftot, 256)), dim3(256), 0, ctx->stream, am, nd, lr, w.out + 3);
     TENRSO_HIP_TRY(hipGetLastError());
