@@ -1537,6 +1537,11 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     const i64 target = (i64)wpc * ctx->num_cus - ((fuse_job && occ <= wpc) ? 1 : 0);
     // each workgroup keeps >= WAVES slow indices
     i64 ssp = std::max<i64>(1, target / std::max<i64>(1, tiles) / a.nwin);
+    // Many row tiles but fewer than 3/4 of the resident slots (9216 rows of 32-row
+    // tiles: 288 workgroups on 512 slots): about 1.5x the slots in workgroups.
+    // Swept on the 9216 x 96 x 96 view of 96^4 at rank 32: 1 split 288
+    // workgroups, 3 splits 864 the fastest (more splits cost their reduction).
+    if (!fuse_job && tiles * a.nwin * ssp * 4 < target * 3) ssp = cdiv(target * 3, 2 * tiles * a.nwin);
     ssp = std::min<i64>(ssp, a.n_slow >= waves ? a.n_slow / waves : 1);
     ssp = std::min<i64>(ssp, 65535 / a.nwin);
     // a caller's workspace bounds the partials: fewer splits, or refuse
