@@ -75,8 +75,12 @@ def test_device_sort_and_csf_bit_exact(shape, nnz):
         assert np.array_equal(got.to_coo().to_dense(), coo.to_dense())
 
 
+# (40, 30, 20) with 600 / 4200 nonzeros: 5 and 33 workers of 128 nonzeros, so a
+# workgroup of 32 workers folds 10, or 64 and then 2, carry slots (no multiple of
+# the 8 slots its fold loads at a time)
 @pytest.mark.parametrize("shape,nnz,R", [((7, 6, 5), 60, 3), ((40, 30, 20), 5000, 32), ((9, 8, 7, 6), 700, 16),
-                                         ((50, 60), 900, 64), ((30, 20, 10), 2000, 70)])
+                                         ((50, 60), 900, 64), ((30, 20, 10), 2000, 70), ((40, 30, 20), 600, 32),
+                                         ((40, 30, 20), 4200, 32)])
 def test_sparse_mttkrp_vs_fp64(shape, nnz, R):
     print(f"[case] sparse mttkrp {shape} nnz={nnz} R={R}", flush=True)
     idx, vals = random_coo(shape, nnz, seed=nnz + R)
