@@ -483,17 +483,30 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
           for (int h = 0; h < 2; ++h)
             b.x[mt][h] = *reinterpret_cast<const floatx4*>(((okm >> (4 * h)) & 1u) ? base + 16 * h : X);
         } else if constexpr (XL == 3) {
-          // an even fastest mode (and row length) on an 8-byte aligned X: pairs,
-          // all in or all out
+          // an even fastest mode (and row length) on an 8-byte aligned X.  A block
+          // wholly below kv (a wave-uniform test): one 16-byte vector per h, as a
+          // copy of 8-byte aligned memory (the compiler emits the 16-byte load,
+          // which this hardware serves at any 4-byte alignment).  The window's
+          // last, partial block: pairs, all in or all out, so that no load
+          // reaches past the end of a row (or of X).
+          if (b.kb * KB + KB <= kv) {
 #pragma unroll
-          for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int pr = 0; pr < 2; ++pr) {
-              const float2 v =
-                  *reinterpret_cast<const float2*>(((okm >> (4 * h + 2 * pr)) & 1u) ? base + 16 * h + 2 * pr : X);
-              b.x[mt][h][2 * pr] = v.x;
-              b.x[mt][h][2 * pr + 1] = v.y;
+            for (int h = 0; h < 2; ++h) {
+              floatx4 v;
+              __builtin_memcpy(&v, __builtin_assume_aligned(base + 16 * h, 8), sizeof(v));
+              b.x[mt][h] = v;
             }
+          } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+              for (int pr = 0; pr < 2; ++pr) {
+                const float2 v =
+                    *reinterpret_cast<const float2*>(((okm >> (4 * h + 2 * pr)) & 1u) ? base + 16 * h + 2 * pr : X);
+                b.x[mt][h][2 * pr] = v.x;
+                b.x[mt][h][2 * pr + 1] = v.y;
+              }
+          }
         } else {
 #pragma unroll
           for (int h = 0; h < 2; ++h)
