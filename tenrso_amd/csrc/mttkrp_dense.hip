@@ -1465,7 +1465,15 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     // ~22 us beside the MTTKRP against ~14 us alone).
     constexpr int WAVES = 4, WAVES_BF16 = 8;
     const int waves = bf16 ? WAVES_BF16 : WAVES;
-    const int MT = (caseB && (I % 2 != 0 || (uintptr_t)Xuse % 8 != 0)) ? 1 : 2;
+    // FP32 at rank <= 32 (NT <= 2), unmasked windows and many rows: 64-row
+    // workgroups (MT = 4).  Each B operand (window read and PRE scaling) then
+    // feeds four row tiles instead of two, and the last mode loads 4 rows per
+    // 16-byte vector instead of 2 per 8-byte one.  The inlined ALS job already
+    // sets these instances' register budget, so the larger tile keeps two
+    // workgroups per CU.
+    const bool mt4 = !bf16 && NT <= 2 && XL == 0 && (I % 64 == 0 || I >= 2048) &&
+                     (!caseB || (I % 4 == 0 && (uintptr_t)Xuse % 16 == 0));
+    const int MT = mt4 ? 4 : (caseB && (I % 2 != 0 || (uintptr_t)Xuse % 8 != 0)) ? 1 : 2;
     const i64 rows_per_wg = 16 * MT;
     const i64 rtiles = (i64)cdiv(I, rows_per_wg);
     const i64 tiles = rtiles * rchunks;
@@ -1482,6 +1490,15 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
         constexpr int L = decltype(xl)::value;  // XL; 2 and 3 only for fp32 case A
         constexpr int LB = L >= 2 ? 1 : L;      // instances that have no element- or pair-wise form
         if (!bf16) {
+          if constexpr (L == 0) {
+            if (MT == 4) {
+              if (!caseB)
+                return NT == 1 ? f(mttkrp_f32_mfma<1, WAVES, 4, false, KWMAX, WV, 0>)
+                               : f(mttkrp_f32_mfma<2, WAVES, 4, false, KWMAX, WV, 0>);
+              return NT == 1 ? f(mttkrp_f32_mfma<1, WAVES, 4, true, KWMAX, WV, 0>)
+                             : f(mttkrp_f32_mfma<2, WAVES, 4, true, KWMAX, WV, 0>);
+            }
+          }
           if (!caseB) {
             if (NT == 1) return f(mttkrp_f32_mfma<1, WAVES, 2, false, KWMAX, WV, L>);
             if (NT == 2) return f(mttkrp_f32_mfma<2, WAVES, 2, false, KWMAX, WV, L>);
