@@ -36,10 +36,13 @@ def _gpu():
 
 
 # shapes: fast MFMA path (fastest other mode % 32 == 0), ragged rows, fallback path,
-# 2-way, 4-way and 5-way tensors
+# 2-way, 4-way and 5-way tensors; 64-row tiles at rank <= 32 (rows a multiple of 64,
+# or >= 2048: mode 0 of (2050, 8, 32), the last mode of (8, 16, 2052) with a partial
+# last tile, both modes 0 and 2 of (128, 96, 64))
 SHAPES = [
     ((2, 3, 4), 2), ((4, 5, 6), 3), ((40, 64, 32), 8), ((17, 32, 64), 5), ((64, 96, 32), 16), ((33, 50, 70), 7),
     ((128, 64), 20), ((24, 32, 16, 32), 6), ((6, 5, 4, 3, 32), 3), ((100, 64, 64), 64), ((20, 32, 32), 70),
+    ((128, 96, 64), 32), ((2050, 8, 32), 16), ((8, 16, 2052), 32),
 ]
 
 
