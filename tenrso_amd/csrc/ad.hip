@@ -45,21 +45,39 @@ static inline unsigned cdiv(i64 a, i64 b) { return (unsigned)((a + b - 1) / b); 
 // B[j, c], j = 0 .. J-1, each group a contiguous range of j.
 constexpr int GR_NE = 64;  // rank bound (SOLVE_RMAX)
 constexpr int GR_T = 1024;
+//
+// One launch can serve two modes (the two modes of one tree pass read the same
+// Y): workgroups [0, g0.rows) take g0, the rest g1 (g1.rows = 0: one mode).
 struct GradTree {
   i64 yrows, J, sa, sb;
   const float* B;
 };
+struct GradRows {
+  int n;            // mode
+  i64 rows;         // I_n, one workgroup per row
+  const float* A;   // A_n
+  float* grad;      // or null
+  double* ip_part;  // or null: <M, A> per row (the last mode)
+  GradTree tr;      // TREE only
+};
 template <bool TREE>
-__global__ __launch_bounds__(GR_T) void grad_rows_kernel(const float* __restrict__ part, int splits, i64 rows,
-                                                         int R, const double* __restrict__ G, int nd, int n,
-                                                         const float* __restrict__ A, float* __restrict__ grad,
-                                                         double* __restrict__ ip_part, GradTree tr) {
+__global__ __launch_bounds__(GR_T) void grad_rows_kernel(const float* __restrict__ part, int splits, int R,
+                                                         const double* __restrict__ G, int nd, GradRows g0,
+                                                         GradRows g1) {
   __shared__ double sq[GR_T];
   __shared__ double sM[GR_NE];
   __shared__ double sA[GR_NE];
+  const bool second = (i64)blockIdx.x >= g0.rows;
+  const GradRows g = second ? g1 : g0;
+  const int n = g.n;
+  const i64 rows = g.rows;
+  const float* __restrict__ A = g.A;
+  float* __restrict__ grad = g.grad;
+  double* __restrict__ ip_part = g.ip_part;
+  const GradTree& tr = g.tr;
   const int cw = R <= 32 ? 32 : 64, ng = GR_T / cw;
   const int t = threadIdx.x, c = t % cw, grp = t / cw;
-  const i64 row = blockIdx.x;
+  const i64 row = second ? (i64)blockIdx.x - g0.rows : (i64)blockIdx.x;
   const bool colok = c < R;
   // this group's rows r of Gamma_n's column c (at most four: R / ng <= 4)
   const int r0 = grp * R / ng, r1 = (grp + 1) * R / ng;
@@ -324,15 +342,22 @@ static int loss_grad_device(tenrso_ctx* ctx, const float* X, int nd, const i64* 
       TENRSO_TRY(mttkrp_dense(ctx, TENRSO_PREC_FP32, X, nullptr, 3, v, Fv, R, half == 0 ? 0 : 2,
                               MttkrpOut::PARTIAL, nullptr, &mr));
       const i64 ya = half == 0 ? I0 : I2, yb = half == 0 ? I1 : I3;  // Y rows = ya x yb
+      // both modes of the pass in one launch (the first: M[i] sums over the
+      // second index of Y; the second: over the first)
+      GradRows gr[2] = {};
+      int ng = 0;
       for (int k = 0; k < 2; ++k) {
         const int n = 2 * half + k;
         float* gn = (grads && grads[n]) ? grads[n] : nullptr;
         const bool last = n == nd - 1;
         if (!gn && !last) continue;
-        // mode 2 half, first: M[i] sums over the second index; second: over the first
-        GradTree tr{ya * yb, k == 0 ? yb : ya, k == 0 ? yb : 1, k == 0 ? 1 : yb, F[k == 0 ? n + 1 : n - 1]};
-        hipLaunchKernelGGL(grad_rows_kernel<true>, dim3((unsigned)shape[n]), dim3(GR_T), 0, ctx->stream, mr.partial,
-                           mr.splits, shape[n], (int)R, w.G, nd, n, F[n], gn, last ? w.part : nullptr, tr);
+        gr[ng++] = GradRows{n, shape[n], F[n], gn, last ? w.part : nullptr,
+                            GradTree{ya * yb, k == 0 ? yb : ya, k == 0 ? yb : 1, k == 0 ? 1 : yb,
+                                     F[k == 0 ? n + 1 : n - 1]}};
+      }
+      if (ng > 0) {
+        hipLaunchKernelGGL(grad_rows_kernel<true>, dim3((unsigned)(gr[0].rows + gr[1].rows)), dim3(GR_T), 0,
+                           ctx->stream, mr.partial, mr.splits, (int)R, w.G, nd, gr[0], gr[1]);
         TENRSO_HIP_TRY(hipGetLastError());
       }
     }
@@ -357,7 +382,8 @@ static int loss_grad_device(tenrso_ctx* ctx, const float* X, int nd, const i64* 
                             nullptr, false, nullptr, 0, 0, pre[n]));
     if (shape[n] == 0) continue;
     hipLaunchKernelGGL(grad_rows_kernel<false>, dim3((unsigned)shape[n]), dim3(GR_T), 0, ctx->stream, mr.partial,
-                       mr.splits, shape[n], (int)R, w.G, nd, n, F[n], gn, last ? w.part : nullptr, GradTree{});
+                       mr.splits, (int)R, w.G, nd, GradRows{n, shape[n], F[n], gn, last ? w.part : nullptr, {}},
+                       GradRows{});
     TENRSO_HIP_TRY(hipGetLastError());
   }
   hipLaunchKernelGGL(loss_kernel, dim3(gnorm ? 1 + nd : 1), dim3(256), 0, ctx->stream, w.G, nd, (int)R, w.part,
