@@ -321,7 +321,8 @@ int loss_work(tenrso_ctx* ctx, int nd, const i64* shape, i64 R, LossWork* w) {
 // grads[n] = A_n Gamma_n - mttkrp(X, A, n); loss into w.out[0] (device, and into
 // *hist when not null), with ||X||^2 already in w.scal[SC_NORM_X].  Per mode: the
 // MTTKRP as split partials, then one fused row kernel; modes without a gradient
-// to form are skipped except the last, whose MTTKRP gives <X, Xhat>.
+// to form are skipped except the last, whose MTTKRP gives <X, Xhat>.  With `tree`
+// (four modes) two partial MTTKRPs serve all four modes instead.
 static int loss_grad_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, const float* const* F, i64 R,
                             float* const* grads, LossWork& w, double* hist, const float* Xt = nullptr,
                             double* gnorm = nullptr, bool tree = false) {
