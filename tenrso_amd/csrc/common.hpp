@@ -97,6 +97,7 @@ enum BufSlot : int {
   SLOT_EIG_PART,
   SLOT_EIG_W,
   SLOT_GATE,
+  SLOT_GRAM_CHUNKS,
   SLOT_COUNT
 };
 

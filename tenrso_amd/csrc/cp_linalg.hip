@@ -162,6 +162,75 @@ __global__ __launch_bounds__(1024) void gram_rows_kernel(const float* __restrict
   gram_rows_body(A, rows, R, G);
 }
 
+// Tall factors (more than GRAM_TALL rows, the sparse modes of 10^4 rows): the
+// row loop above runs each thread over rows / 16 rows, one memory round trip
+// after another (62-102 us at 6000-10000 rows, rank 32).  Here the rows go in
+// chunks of GRAM_CH, one workgroup per chunk: the chunk is staged in LDS and
+// thread t forms entries t, t + 1024, .. of the chunk's R x R partial in FP64, in
+// row order; a second launch adds each entry's chunk partials in chunk order
+// (16 thread groups over contiguous chunk ranges, their sums added in group
+// order).  Entries (r, c) and (c, r) add the same products in the same order, so
+// the Gram stays exactly symmetric.
+constexpr int GRAM_CH = 128;
+constexpr i64 GRAM_TALL = 1024;
+__global__ __launch_bounds__(1024) void gram_chunk_kernel(const float* __restrict__ A, i64 rows, int R,
+                                                          double* __restrict__ part) {
+  __shared__ float sA[GRAM_CH * SOLVE_RMAX];
+  const i64 i0 = (i64)blockIdx.x * GRAM_CH;
+  const int n = (int)((rows - i0) < GRAM_CH ? (rows - i0) : GRAM_CH);
+  for (int e = threadIdx.x; e < n * R; e += 1024) sA[e] = A[i0 * R + e];
+  __syncthreads();
+  const int RR = R * R;
+  for (int e = threadIdx.x; e < RR; e += 1024) {
+    const int r = e / R, c = e - (e / R) * R;
+    double a0 = 0.0, a1 = 0.0;
+    int i = 0;
+    for (; i + 2 <= n; i += 2) {
+      a0 += (double)sA[i * R + r] * (double)sA[i * R + c];
+      a1 += (double)sA[(i + 1) * R + r] * (double)sA[(i + 1) * R + c];
+    }
+    if (i < n) a0 += (double)sA[i * R + r] * (double)sA[i * R + c];
+    part[(i64)blockIdx.x * RR + e] = a0 + a1;
+  }
+}
+
+__global__ __launch_bounds__(1024) void gram_chunk_sum_kernel(const double* __restrict__ part, int nch, int RR,
+                                                              double* __restrict__ G) {
+  __shared__ double sq[1024];
+  const int t = threadIdx.x, j = t & 63, grp = t >> 6;
+  const int e = blockIdx.x * 64 + j;
+  const int k0 = grp * nch / 16, k1 = (grp + 1) * nch / 16;
+  double s0 = 0.0, s1 = 0.0;
+  if (e < RR) {
+    int k = k0;
+    for (; k + 2 <= k1; k += 2) {
+      s0 += part[(i64)k * RR + e];
+      s1 += part[(i64)(k + 1) * RR + e];
+    }
+    if (k < k1) s0 += part[(i64)k * RR + e];
+  }
+  sq[t] = s0 + s1;
+  __syncthreads();
+  if (grp == 0 && e < RR) {
+    double s = 0.0;
+#pragma unroll
+    for (int g = 0; g < 16; ++g) s += sq[g * 64 + j];
+    G[e] = s;
+  }
+}
+
+static int gram_tall(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G, hipStream_t stream) {
+  const i64 nch = (rows + GRAM_CH - 1) / GRAM_CH;
+  void* pp = nullptr;
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_GRAM_CHUNKS, (size_t)nch * R * R * sizeof(double), &pp));
+  hipLaunchKernelGGL(gram_chunk_kernel, dim3((unsigned)nch), dim3(1024), 0, stream, A, rows, (int)R, (double*)pp);
+  TENRSO_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(gram_chunk_sum_kernel, dim3(cdiv(R * R, 64)), dim3(1024), 0, stream, (const double*)pp,
+                     (int)nch, (int)(R * R), G);
+  TENRSO_HIP_TRY(hipGetLastError());
+  return TENRSO_OK;
+}
+
 // The Grams of several factors in one launch: grid (R, count), G_k at G + k*R*R.
 struct GramMany {
   const float* A[MAXD];
@@ -178,6 +247,13 @@ int gram_f64_many(tenrso_ctx* ctx, int count, const float* const* A, const i64* 
     return TENRSO_ERR_UNSUPPORTED;
   }
   if (R == 0 || count == 0) return TENRSO_OK;
+  bool tall = false;
+  for (int k = 0; k < count; ++k) tall = tall || rows[k] > GRAM_TALL;
+  if (tall) {
+    // the chunked form per factor (its chunk partials share one buffer, in stream order)
+    for (int k = 0; k < count; ++k) TENRSO_TRY(gram_f64(ctx, A[k], rows[k], R, G + (i64)k * R * R));
+    return TENRSO_OK;
+  }
   GramMany m{};
   for (int k = 0; k < count; ++k) {
     m.A[k] = A[k];
@@ -194,6 +270,7 @@ int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G, hipStr
     return TENRSO_ERR_UNSUPPORTED;
   }
   if (R == 0) return TENRSO_OK;
+  if (rows > GRAM_TALL) return gram_tall(ctx, A, rows, R, G, stream ? stream : ctx->stream);
   hipLaunchKernelGGL(gram_rows_kernel, dim3((unsigned)R), dim3(1024), 0, stream ? stream : ctx->stream, A, rows,
                      (int)R, G);
   TENRSO_HIP_TRY(hipGetLastError());
