@@ -98,6 +98,7 @@ enum BufSlot : int {
   SLOT_EIG_W,
   SLOT_GATE,
   SLOT_GRAM_CHUNKS,
+  SLOT_XPAD,
   SLOT_COUNT
 };
 

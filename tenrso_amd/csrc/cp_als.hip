@@ -366,10 +366,30 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
     Xt = pt;
     return TENRSO_OK;
   };
+  // FP32 with a last mode of no multiple of 32 floats (250^3, say): its rows are
+  // not 128-byte aligned, so every 16-byte vector run straddles cache lines, and
+  // the last mode is i-contiguous and ragged.  Where padding adds at most 1/8,
+  // modes 0..N-2 contract a copy with the last mode padded to a multiple of 32
+  // (zeros) and the last mode the last-to-front copy, its own last mode padded
+  // the same way.  One pass each per call.
+  const i64 Kx32 = (Kx + 31) / 32 * 32, Bt32 = (Bt + 31) / 32 * 32;
+  const bool fpad = !bf16 && nd >= 2 && total > 0 && Kx32 != Kx && Kx32 * 8 <= Kx * 9 && Bt32 * 8 <= Bt * 9;
+  if (fpad) {
+    pb.prepare = [&]() -> int {
+      void *pp, *pt;
+      TENRSO_TRY(ctx_buffer(ctx, SLOT_XPAD, (size_t)(total / Kx * Kx32) * sizeof(float), &pp));
+      TENRSO_TRY(ctx_buffer(ctx, SLOT_XBF16_T, (size_t)(total / Bt * Bt32) * sizeof(float), &pt));
+      TENRSO_TRY(copy_f32_padded(ctx, X, nd, shape, Kx32, (float*)pp));
+      TENRSO_TRY(transpose_last_to_front_f32_padded(ctx, X, nd, shape, Bt32, (float*)pt));
+      Xk = pp;
+      Xt = pt;
+      return TENRSO_OK;
+    };
+  }
+  const i64 x_pad = padded ? Kxp : (fpad ? Kx32 : 0), xt_pad = padded ? Btp : (fpad ? Bt32 : 0);
   pb.mttkrp = [&](int mode, const float* const* Fc, const AlsJob* job, MttkrpResult* mr) {
     return mttkrp_dense(ctx, bf16 ? TENRSO_PREC_BF16 : TENRSO_PREC_FP32, Xk, Xt, nd, shape, Fc, R, mode,
-                        MttkrpOut::PARTIAL, nullptr, mr, job, /*xt_f32=*/true, nullptr, padded ? Kxp : 0,
-                        padded ? Btp : 0);
+                        MttkrpOut::PARTIAL, nullptr, mr, job, /*xt_f32=*/true, nullptr, x_pad, xt_pad);
   };
   return cp_als_run(ctx, pb, F, o, res, hist);
 }

@@ -86,6 +86,8 @@ int transpose_last_to_front_f32(tenrso_ctx* ctx, const float* src, int nd, const
 // copy with its last mode padded to Kp, and the fp32 last-to-front copy with
 // its new last mode (shape[nd-2]) padded to Bp; pad entries are zero.
 int convert_bf16_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, i64 Kp, uint16_t* dst);
+// The fp32 tensor with its last mode padded to Kp (the FP32 ALS of ragged shapes).
+int copy_f32_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, i64 Kp, float* dst);
 int transpose_last_to_front_f32_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, i64 Bp,
                                        float* dst);
 

@@ -459,8 +459,9 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
     // XL > 0: elements at k >= kv are not loaded (their address becomes X) and
     // are zeroed in compute(), where the value is used anyway (a select here
     // would wait for the load and drain the prefetch)
+    // XL 4 (a zero-padded copy): no masks, the pads are zero
     uint32_t okm = 0xffu;
-    if constexpr (XL > 0) {
+    if constexpr (XL > 0 && XL != 4) {
       const int k0 = b.kb * KB + 4 * g;
       okm = 0;
 #pragma unroll
@@ -474,7 +475,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         const float* base = ubase + (live ? loff[mt] : 0);
-        if constexpr (XL == 0) {
+        if constexpr (XL == 0 || XL == 4) {
           b.x[mt][0] = *reinterpret_cast<const floatx4*>(base);
           b.x[mt][1] = *reinterpret_cast<const floatx4*>(base + 16);
         } else if constexpr (XL == 1) {
@@ -547,7 +548,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_f32_mfma(const float* __res
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         xm[mt] = b.x[mt][h];
-        if constexpr (XL > 0) {
+        if constexpr (XL > 0 && XL != 4) {
 #pragma unroll
           for (int sidx = 0; sidx < 4; ++sidx)
             if (!((b.okm >> (4 * h + sidx)) & 1u)) xm[mt][sidx] = 0.f;
@@ -1118,11 +1119,6 @@ __global__ __launch_bounds__(256) void f32_to_bf16_pad_kernel(const float* __res
   dst[e] = k < K ? f32_to_bf16_rne(src[r * K + k]) : (uint16_t)0;
 }
 
-__global__ __launch_bounds__(256) void zero_u32_kernel(uint32_t* __restrict__ p, i64 n) {
-  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (e < n) p[e] = 0u;
-}
-
 int convert_bf16_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, i64 Kp, uint16_t* dst) {
   i64 total = 1;
   for (int k = 0; k < nd; ++k) total *= shape[k];
@@ -1135,16 +1131,43 @@ int convert_bf16_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* sh
   return TENRSO_OK;
 }
 
+// one workgroup per row (no division per element)
+__global__ __launch_bounds__(256) void f32_pad_kernel(const float* __restrict__ src, i64 K, i64 Kp,
+                                                      float* __restrict__ dst) {
+  const i64 r = blockIdx.x;
+  for (i64 k = threadIdx.x; k < Kp; k += 256) dst[r * Kp + k] = k < K ? src[r * K + k] : 0.0f;
+}
+
+// zeros at [B, Bp) of each of n rows of Bp (the pads of a padded copy, nothing else)
+__global__ __launch_bounds__(256) void zero_pads_kernel(float* __restrict__ dst, i64 n, i64 B, i64 Bp) {
+  const i64 np = Bp - B;
+  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n * np) return;
+  const i64 r = e / np;
+  dst[r * Bp + B + (e - r * np)] = 0.0f;
+}
+
+int copy_f32_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, i64 Kp, float* dst) {
+  i64 total = 1;
+  for (int k = 0; k < nd; ++k) total *= shape[k];
+  const i64 K = shape[nd - 1];
+  if (total == 0) return TENRSO_OK;
+  const i64 rows = total / K;
+  hipLaunchKernelGGL(f32_pad_kernel, dim3((unsigned)rows), dim3(256), 0, ctx->stream, src, K, Kp, dst);
+  TENRSO_HIP_TRY(hipGetLastError());
+  return TENRSO_OK;
+}
+
 int transpose_last_to_front_f32_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, i64 Bp,
                                        float* dst) {
   i64 total = 1;
   for (int k = 0; k < nd; ++k) total *= shape[k];
   if (total == 0) return TENRSO_OK;
   const i64 K = shape[nd - 1], rows = total / K, B = shape[nd - 2];
-  const i64 out_n = K * (rows / B) * Bp;
+  const i64 nrow = K * (rows / B);  // rows of Bp in the copy
   if (Bp != B)
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(cdiv(out_n, 256)), dim3(256), 0, ctx->stream,
-                       reinterpret_cast<uint32_t*>(dst), out_n);
+    hipLaunchKernelGGL(zero_pads_kernel, dim3(cdiv(nrow * (Bp - B), 256)), dim3(256), 0, ctx->stream, dst, nrow, B,
+                       Bp);
   hipLaunchKernelGGL((transpose_kernel<float, float>), dim3(cdiv(K, 32), cdiv(rows, 32)), dim3(256), 0, ctx->stream,
                      src, rows, K, dst, B, Bp);
   TENRSO_HIP_TRY(hipGetLastError());
@@ -1407,7 +1430,7 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
   // memory extent of the fastest other mode: the BF16 ALS copies of a tensor
   // whose last mode is no multiple of the load vector have that mode padded
   // (zeros), so every row starts on a vector boundary
-  const i64 pad = !bf16 ? 0 : (mode == nd - 1 && nd >= 2 ? xt_pad : x_pad);
+  const i64 pad = mode == nd - 1 && nd >= 2 ? xt_pad : x_pad;
   const i64 fast_stride = (pad > 0 && !caseB) ? pad : pl.fast_size;
   const i64 Qmem = (pad > 0 && !caseB) ? pl.Q / pl.fast_size * pad : pl.Q;
 
@@ -1449,6 +1472,11 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
   // bf16 kernels need 16-byte alignment, else the generic kernel runs.
   const bool aligned16 = (uintptr_t)Xuse % 16 == 0;
   if (!bf16 && !caseB && !aligned16 && kw > 0) XL = 2;
+  // FP32 case A on a zero-padded copy whose rows reach a multiple of 32: every
+  // window's last block ends inside the padded row, so X loads need no masks
+  // (the pads are zero, and window rows past kv are staged as zeros); only the
+  // window load keeps kv (XL 4)
+  if (!bf16 && !caseB && pad > 0 && XL == 1 && aligned16 && fast_stride >= (pl.fast_size + 31) / 32 * 32) XL = 4;
   // element-wise fp32 loads become pairs (8 bytes) where every row start is even
   // and X is 8-byte aligned (a fastest mode of 250, say)
   if (XL == 2 && fast_stride % 2 == 0 && Qmem % 2 == 0 && (uintptr_t)Xuse % 8 == 0) XL = 3;
@@ -1487,7 +1515,7 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
     auto with_kernel = [&](auto&& f) {
       auto pick = [&](auto wv, auto xl) {
         constexpr bool WV = decltype(wv)::value;
-        constexpr int L = decltype(xl)::value;  // XL; 2 and 3 only for fp32 case A
+        constexpr int L = decltype(xl)::value;  // XL; 2, 3 and 4 only for fp32 case A
         constexpr int LB = L >= 2 ? 1 : L;      // instances that have no element- or pair-wise form
         if (!bf16) {
           if constexpr (L == 0) {
@@ -1530,6 +1558,7 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
       auto pick_xl = [&](auto wv) {
         if (XL == 0) return pick(wv, std::integral_constant<int, 0>{});
         if (XL == 1) return pick(wv, std::integral_constant<int, 1>{});
+        if (XL == 4) return pick(wv, std::integral_constant<int, 4>{});
         if (XL == 3) return pick(wv, std::integral_constant<int, 3>{});
         return pick(wv, std::integral_constant<int, 2>{});
       };
