@@ -328,7 +328,10 @@ int cp_als_device(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, flo
   // padded with zeros (ragged shapes such as 250^3), so every mode still runs
   // on the matrix cores.
   const i64 Kx = shape[nd - 1], Bt = nd >= 2 ? shape[nd - 2] : 1;
-  const i64 Kxp = (Kx + 7) / 8 * 8, Btp = (Bt + 3) / 4 * 4;
+  // (the last-to-front copy's mode goes to a multiple of 32 where that adds at
+  // most 1/8: 128-byte rows, and no masked loads)
+  const i64 Kxp = (Kx + 7) / 8 * 8, Bt32a = (Bt + 31) / 32 * 32;
+  const i64 Btp = Bt32a * 8 <= Bt * 9 ? Bt32a : (Bt + 3) / 4 * 4;
   const bool padded = bf16 && nd >= 2 && total > 0 && (Kxp != Kx || Btp != Bt);
   auto bf16_buffers = [&](void** pb16, void** pt) -> int {
     const size_t nb = padded ? (size_t)(total / Kx * Kxp) : (size_t)total;

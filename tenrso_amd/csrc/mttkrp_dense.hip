@@ -710,9 +710,10 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __rest
     for (int nt = 0; nt < NT; ++nt) b.pre[nt] = prow[poff[nt]];
     const bool live = t < T;  // past the end: one cache line at X (see the FP32 kernel)
     const XIN* ubase = X + (live ? cur.offset(a) : 0);
-    // XL == 1: vectors at k >= kv are loaded from X instead and zeroed in compute()
+    // XL == 1: vectors at k >= kv are loaded from X instead and zeroed in compute();
+    // XL == 4 (a zero-padded copy): no masks
     uint32_t okm = 0xffu;
-    if constexpr (XL > 0) {
+    if constexpr (XL > 0 && XL != 4) {
       okm = 0;
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks)
@@ -748,7 +749,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __rest
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
           floatx8 xv = b.x[mt][ks];
-          if constexpr (XL > 0) {
+          if constexpr (XL > 0 && XL != 4) {
 #pragma unroll
             for (int e = 0; e < 8; ++e)
               if (!((b.okm >> (2 * ks + e / 4)) & 1u)) xv[e] = 0.f;
@@ -786,7 +787,7 @@ __global__ __launch_bounds__(WAVES * 64) void mttkrp_bf16_mfma(const XIN* __rest
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
           xv[mt] = b.x[mt][ks];
-          if constexpr (XL > 0) {
+          if constexpr (XL > 0 && XL != 4) {
             if (!((b.okm >> (2 * ks)) & 1u)) xv[mt] = bf16x8{};
           }
         }
@@ -1111,12 +1112,11 @@ int convert_bf16(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, bo
 }
 
 // [rows, K] fp32 -> [rows, Kp] bf16 (RNE), zero in the pad columns k >= K
-__global__ __launch_bounds__(256) void f32_to_bf16_pad_kernel(const float* __restrict__ src, i64 rows, i64 K,
-                                                              i64 Kp, uint16_t* __restrict__ dst) {
-  const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
-  if (e >= rows * Kp) return;
-  const i64 r = e / Kp, k = e - r * Kp;
-  dst[e] = k < K ? f32_to_bf16_rne(src[r * K + k]) : (uint16_t)0;
+// one workgroup per row (no division per element)
+__global__ __launch_bounds__(256) void f32_to_bf16_pad_kernel(const float* __restrict__ src, i64 K, i64 Kp,
+                                                              uint16_t* __restrict__ dst) {
+  const i64 r = blockIdx.x;
+  for (i64 k = threadIdx.x; k < Kp; k += 256) dst[r * Kp + k] = k < K ? f32_to_bf16_rne(src[r * K + k]) : (uint16_t)0;
 }
 
 int convert_bf16_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* shape, i64 Kp, uint16_t* dst) {
@@ -1125,8 +1125,7 @@ int convert_bf16_padded(tenrso_ctx* ctx, const float* src, int nd, const i64* sh
   const i64 K = shape[nd - 1];
   if (total == 0) return TENRSO_OK;
   const i64 rows = total / K;
-  hipLaunchKernelGGL(f32_to_bf16_pad_kernel, dim3(cdiv(rows * Kp, 256)), dim3(256), 0, ctx->stream, src, rows, K, Kp,
-                     dst);
+  hipLaunchKernelGGL(f32_to_bf16_pad_kernel, dim3((unsigned)rows), dim3(256), 0, ctx->stream, src, K, Kp, dst);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -1472,17 +1471,17 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
   // bf16 kernels need 16-byte alignment, else the generic kernel runs.
   const bool aligned16 = (uintptr_t)Xuse % 16 == 0;
   if (!bf16 && !caseB && !aligned16 && kw > 0) XL = 2;
-  // FP32 case A on a zero-padded copy whose rows reach a multiple of 32: every
+  // Case A on a zero-padded copy whose rows reach a multiple of 32: every
   // window's last block ends inside the padded row, so X loads need no masks
   // (the pads are zero, and window rows past kv are staged as zeros); only the
   // window load keeps kv (XL 4)
-  if (!bf16 && !caseB && pad > 0 && XL == 1 && aligned16 && fast_stride >= (pl.fast_size + 31) / 32 * 32) XL = 4;
+  if (!caseB && pad > 0 && XL == 1 && aligned16 && fast_stride >= (pl.fast_size + 31) / 32 * 32) XL = 4;
   // element-wise fp32 loads become pairs (8 bytes) where every row start is even
   // and X is 8-byte aligned (a fastest mode of 250, say)
   if (XL == 2 && fast_stride % 2 == 0 && Qmem % 2 == 0 && (uintptr_t)Xuse % 8 == 0) XL = 3;
   const bool mfma_ok = pl.n_oth >= 1 && pl.J > 0 && kw > 0 && !(bf16 && caseB) && I > 0 &&
                        (uintptr_t)Xuse % 4 == 0 &&
-                       (bf16 ? aligned16 && XL < 2 : (caseB || XL >= 2 || aligned16));
+                       (bf16 ? aligned16 && (XL < 2 || XL == 4) : (caseB || XL >= 2 || aligned16));
 
   float* partial = nullptr;
   int splits = 1;
@@ -1517,6 +1516,7 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
         constexpr bool WV = decltype(wv)::value;
         constexpr int L = decltype(xl)::value;  // XL; 2, 3 and 4 only for fp32 case A
         constexpr int LB = L >= 2 ? 1 : L;      // instances that have no element- or pair-wise form
+        constexpr int LX = L == 4 ? 4 : LB;     // bf16 instances: also the unmasked padded form
         if (!bf16) {
           if constexpr (L == 0) {
             if (MT == 4) {
@@ -1542,18 +1542,18 @@ int mttkrp_dense(tenrso_ctx* ctx, int prec, const void* X, const void* X_t, int 
           return f(mttkrp_f32_mfma<4, WAVES, 1, true, KWMAX, WV, LB>);
         }
         if (x3) {
-          if (NT == 1) return f(mttkrp_bf16_mfma<float, 1, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
-          if (NT == 2) return f(mttkrp_bf16_mfma<float, 2, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
-          return f(mttkrp_bf16_mfma<float, 4, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
+          if (NT == 1) return f(mttkrp_bf16_mfma<float, 1, WAVES_BF16, 2, 1, KWMAX, WV, LX>);
+          if (NT == 2) return f(mttkrp_bf16_mfma<float, 2, WAVES_BF16, 2, 1, KWMAX, WV, LX>);
+          return f(mttkrp_bf16_mfma<float, 4, WAVES_BF16, 2, 1, KWMAX, WV, LX>);
         }
         if (KB == 64) {  // only a fastest mode divisible by 64 (never ragged)
           if (NT == 1) return f(mttkrp_bf16_mfma<uint16_t, 1, WAVES_BF16, 2, 2, KWMAX, WV, 0>);
           if (NT == 2) return f(mttkrp_bf16_mfma<uint16_t, 2, WAVES_BF16, 2, 2, KWMAX, WV, 0>);
           return f(mttkrp_bf16_mfma<uint16_t, 4, WAVES_BF16, 2, 2, KWMAX, WV, 0>);
         }
-        if (NT == 1) return f(mttkrp_bf16_mfma<uint16_t, 1, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
-        if (NT == 2) return f(mttkrp_bf16_mfma<uint16_t, 2, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
-        return f(mttkrp_bf16_mfma<uint16_t, 4, WAVES_BF16, 2, 1, KWMAX, WV, LB>);
+        if (NT == 1) return f(mttkrp_bf16_mfma<uint16_t, 1, WAVES_BF16, 2, 1, KWMAX, WV, LX>);
+        if (NT == 2) return f(mttkrp_bf16_mfma<uint16_t, 2, WAVES_BF16, 2, 1, KWMAX, WV, LX>);
+        return f(mttkrp_bf16_mfma<uint16_t, 4, WAVES_BF16, 2, 1, KWMAX, WV, LX>);
       };
       auto pick_xl = [&](auto wv) {
         if (XL == 0) return pick(wv, std::integral_constant<int, 0>{});
