@@ -508,10 +508,16 @@ def test_cp_als_bf16_start_gate_changes_timing_only(monkeypatch):
 # (2500, 24, 20), (1500, 40, 36): a mode of more than 1024 rows takes the chunked
 # Gram (128-row chunks, their FP64 partials summed in order), at rank 8 and at
 # rank 36 (R^2 > 1024: threads form two entries each).  (40, 64, 250),
-# (96, 250, 250): a ragged last mode, which FP32 runs on copies padded to a
-# multiple of 32 (and BF16 on its own padded copies).
+# (96, 250, 250), (12, 10, 64, 250), (300, 250): a ragged last mode, which FP32
+# runs on copies padded to a multiple of 32 (and BF16 on its own padded copies),
+# in 3-, 4- and 2-way tensors (the 2-way one pads its transposed copy too).  The
+# 4-way case runs at rank 4, where eight sweeps fit to 0.994: at rank 6 that shape
+# fits only 0.77 after eight sweeps, and BF16's rounding of X then moves the factors
+# by 3e-4 relative whether the last mode is ragged (250) or not (248), which says
+# more about the conditioning than about the padded copies.
 ALS_NWAY = [((64, 48), 5), ((96, 32), 8), ((16, 20, 12, 32), 6), ((6, 5, 8, 7, 32), 3), ((4, 5, 3, 6, 4, 32), 3),
-            ((30, 20, 25), 1), ((2500, 24, 20), 8), ((1500, 40, 36), 36), ((40, 64, 250), 8), ((96, 250, 250), 8)]
+            ((30, 20, 25), 1), ((2500, 24, 20), 8), ((1500, 40, 36), 36), ((40, 64, 250), 8), ((96, 250, 250), 8),
+            ((12, 10, 64, 250), 4), ((300, 250), 7)]
 
 
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
