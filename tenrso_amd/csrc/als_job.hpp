@@ -45,7 +45,7 @@ struct AlsJob {
   double l2;
   int tik_order;
   double tik_lambda;
-  double* Winv;          // 64 x 64, stride 64
+  double* Winv;          // 64 x 64, stride 64 (128 x 128, stride 128, in the wide-rank job)
   double* scal;          // scalar block
   int status_slot;       // scal index that receives max(path)
   int status_reset;      // write path instead of max(previous, path)

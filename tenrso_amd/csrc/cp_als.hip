@@ -65,8 +65,13 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
     set_error("tensors with more than %d modes are not supported", MAXD);
     return TENRSO_ERR_UNSUPPORTED;
   }
-  if (R > JOB_NE) {
-    set_error("rank %lld exceeds the on-device normal-equation solve bound %d", (long long)R, JOB_NE);
+  // ranks above JOB_NE take the wide-rank job and row solve (cp_linalg.hip)
+  if (R > WIDE_NE) {
+    set_error("rank %lld exceeds the on-device normal-equation solve bound %d", (long long)R, WIDE_NE);
+    return TENRSO_ERR_UNSUPPORTED;
+  }
+  if (o->orthogonal && R > SOLVE_RMAX) {
+    set_error("the orthogonal constraint supports rank <= %d on the device (rank %lld)", SOLVE_RMAX, (long long)R);
     return TENRSO_ERR_UNSUPPORTED;
   }
   TENRSO_TRY(set_device(ctx));
@@ -80,7 +85,8 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
   void *p_scal, *p_G, *p_Winv, *p_tmp, *p_set;
   TENRSO_TRY(ctx_buffer(ctx, SLOT_SCALARS, SC_COUNT * sizeof(double), &p_scal));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_GRAMS, (size_t)(nd + 1) * R * R * sizeof(double), &p_G));
-  TENRSO_TRY(ctx_buffer(ctx, SLOT_SOLVE, (size_t)JOB_NE * JOB_NE * sizeof(double), &p_Winv));
+  const i64 ne = R > JOB_NE ? WIDE_NE : JOB_NE;  // order (and stride) of Winv
+  TENRSO_TRY(ctx_buffer(ctx, SLOT_SOLVE, (size_t)ne * ne * sizeof(double), &p_Winv));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_IP, (size_t)std::max<i64>(4096, maxI + 16) * sizeof(double), &p_tmp));
   TENRSO_TRY(ctx_buffer(ctx, SLOT_FSET, (size_t)ftot * sizeof(float), &p_set));
   double* scal = (double*)p_scal;

@@ -1,7 +1,8 @@
 // cp_linalg.hip -- FP64 kernels of the ALS step on gfx950 besides the MTTKRP.
 //
 //   Gram           G = A^T A                                   (cp/helpers.rs:409-452)
-//   ALS job        Gamma_n^{-1} and the fit as one workgroup (als_job.hpp)
+//   ALS job        Gamma_n^{-1} and the fit as one workgroup (als_job.hpp; ranks
+//                  above 64 in the wide-rank job below)
 //   row solve      A_n[i,:] = M[i,:] Gamma_n^{-1} for every row (cp/helpers.rs:455-495)
 //
 // All of it runs in FP64 on the device; the factors stay FP32.  Every reduction
@@ -131,15 +132,18 @@ int tensor_norm_min(tenrso_ctx* ctx, const float* X, i64 n, double* tmp, double*
 
 // ---------------------------------------------------------------- Gram A^T A
 // grid: R workgroups of 1024 threads; workgroup r computes Gram row r.  Thread
-// (ch, c) sums A[i][r] A[i][c] in FP64 over row chunk ch (16 chunks), with all of
-// its loads independent; the 16 chunk sums are added in fixed order.
+// (ch, c) sums A[i][r] A[i][c] in FP64 over row chunk ch (1024 / NC chunks: 16
+// at rank <= 64, 8 at ranks up to WIDE_NE), with all of its loads independent;
+// the chunk sums are added in fixed order.
+template <int NC>
 __device__ __forceinline__ void gram_rows_body(const float* __restrict__ A, i64 rows, int R, double* __restrict__ G) {
-  __shared__ double part[16 * SOLVE_RMAX];
-  const int c = threadIdx.x & 63, ch = threadIdx.x >> 6;
+  constexpr int NG = 1024 / NC;
+  __shared__ double part[1024];
+  const int c = threadIdx.x % NC, ch = threadIdx.x / NC;
   const int r = blockIdx.x;
   double acc0 = 0.0, acc1 = 0.0;
   if (c < R) {
-    const i64 i0 = (i64)ch * rows / 16, i1 = (i64)(ch + 1) * rows / 16;
+    const i64 i0 = (i64)ch * rows / NG, i1 = (i64)(ch + 1) * rows / NG;
     i64 i = i0;
     for (; i + 2 <= i1; i += 2) {
       acc0 += (double)A[i * R + r] * (double)A[i * R + c];
@@ -147,19 +151,20 @@ __device__ __forceinline__ void gram_rows_body(const float* __restrict__ A, i64 
     }
     if (i < i1) acc0 += (double)A[i * R + r] * (double)A[i * R + c];
   }
-  part[ch * SOLVE_RMAX + c] = acc0 + acc1;
+  part[ch * NC + c] = acc0 + acc1;
   __syncthreads();
   if (ch == 0 && c < R) {
     double s = 0.0;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) s += part[k * SOLVE_RMAX + c];
+    for (int k = 0; k < NG; ++k) s += part[k * NC + c];
     G[(i64)r * R + c] = s;
   }
 }
 
+template <int NC>
 __global__ __launch_bounds__(1024) void gram_rows_kernel(const float* __restrict__ A, i64 rows, int R,
                                                          double* __restrict__ G) {
-  gram_rows_body(A, rows, R, G);
+  gram_rows_body<NC>(A, rows, R, G);
 }
 
 // Tall factors (more than GRAM_TALL rows, the sparse modes of 10^4 rows): the
@@ -170,14 +175,15 @@ __global__ __launch_bounds__(1024) void gram_rows_kernel(const float* __restrict
 // row order; a second launch adds each entry's chunk partials in chunk order
 // (16 thread groups over contiguous chunk ranges, their sums added in group
 // order).  Entries (r, c) and (c, r) add the same products in the same order, so
-// the Gram stays exactly symmetric.
+// the Gram stays exactly symmetric.  Above rank SOLVE_RMAX a chunk holds half as
+// many rows, so that it fills the same LDS.
 constexpr int GRAM_CH = 128;
 constexpr i64 GRAM_TALL = 1024;
-__global__ __launch_bounds__(1024) void gram_chunk_kernel(const float* __restrict__ A, i64 rows, int R,
+__global__ __launch_bounds__(1024) void gram_chunk_kernel(const float* __restrict__ A, i64 rows, int R, int chr,
                                                           double* __restrict__ part) {
-  __shared__ float sA[GRAM_CH * SOLVE_RMAX];
-  const i64 i0 = (i64)blockIdx.x * GRAM_CH;
-  const int n = (int)((rows - i0) < GRAM_CH ? (rows - i0) : GRAM_CH);
+  __shared__ float sA[GRAM_CH * SOLVE_RMAX];  // chr * R <= GRAM_CH * SOLVE_RMAX
+  const i64 i0 = (i64)blockIdx.x * chr;
+  const int n = (int)((rows - i0) < chr ? (rows - i0) : chr);
   for (int e = threadIdx.x; e < n * R; e += 1024) sA[e] = A[i0 * R + e];
   __syncthreads();
   const int RR = R * R;
@@ -220,10 +226,13 @@ __global__ __launch_bounds__(1024) void gram_chunk_sum_kernel(const double* __re
 }
 
 static int gram_tall(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G, hipStream_t stream) {
-  const i64 nch = (rows + GRAM_CH - 1) / GRAM_CH;
+  // rows per chunk: the chunk (chr x R fp32) fills the same LDS at every rank
+  const int chr = R <= SOLVE_RMAX ? GRAM_CH : GRAM_CH * SOLVE_RMAX / WIDE_NE;
+  const i64 nch = (rows + chr - 1) / chr;
   void* pp = nullptr;
   TENRSO_TRY(ctx_buffer(ctx, SLOT_GRAM_CHUNKS, (size_t)nch * R * R * sizeof(double), &pp));
-  hipLaunchKernelGGL(gram_chunk_kernel, dim3((unsigned)nch), dim3(1024), 0, stream, A, rows, (int)R, (double*)pp);
+  hipLaunchKernelGGL(gram_chunk_kernel, dim3((unsigned)nch), dim3(1024), 0, stream, A, rows, (int)R, chr,
+                     (double*)pp);
   TENRSO_HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(gram_chunk_sum_kernel, dim3(cdiv(R * R, 64)), dim3(1024), 0, stream, (const double*)pp,
                      (int)nch, (int)(R * R), G);
@@ -236,14 +245,15 @@ struct GramMany {
   const float* A[MAXD];
   i64 rows[MAXD];
 };
+template <int NC>
 __global__ __launch_bounds__(1024) void gram_many_kernel(GramMany m, int R, double* __restrict__ G) {
   const int k = blockIdx.y;
-  gram_rows_body(m.A[k], m.rows[k], R, G + (i64)k * R * R);
+  gram_rows_body<NC>(m.A[k], m.rows[k], R, G + (i64)k * R * R);
 }
 
 int gram_f64_many(tenrso_ctx* ctx, int count, const float* const* A, const i64* rows, i64 R, double* G) {
-  if (R > SOLVE_RMAX) {
-    set_error("rank %lld exceeds the on-device Gram bound %d", (long long)R, SOLVE_RMAX);
+  if (R > WIDE_NE) {
+    set_error("rank %lld exceeds the on-device Gram bound %d", (long long)R, WIDE_NE);
     return TENRSO_ERR_UNSUPPORTED;
   }
   if (R == 0 || count == 0) return TENRSO_OK;
@@ -259,20 +269,29 @@ int gram_f64_many(tenrso_ctx* ctx, int count, const float* const* A, const i64* 
     m.A[k] = A[k];
     m.rows[k] = rows[k];
   }
-  hipLaunchKernelGGL(gram_many_kernel, dim3((unsigned)R, (unsigned)count), dim3(1024), 0, ctx->stream, m, (int)R, G);
+  if (R <= SOLVE_RMAX)
+    hipLaunchKernelGGL(gram_many_kernel<64>, dim3((unsigned)R, (unsigned)count), dim3(1024), 0, ctx->stream, m,
+                       (int)R, G);
+  else
+    hipLaunchKernelGGL(gram_many_kernel<WIDE_NE>, dim3((unsigned)R, (unsigned)count), dim3(1024), 0, ctx->stream, m,
+                       (int)R, G);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
 
 int gram_f64(tenrso_ctx* ctx, const float* A, i64 rows, i64 R, double* G, hipStream_t stream) {
-  if (R > SOLVE_RMAX) {
-    set_error("rank %lld exceeds the on-device Gram bound %d", (long long)R, SOLVE_RMAX);
+  if (R > WIDE_NE) {
+    set_error("rank %lld exceeds the on-device Gram bound %d", (long long)R, WIDE_NE);
     return TENRSO_ERR_UNSUPPORTED;
   }
   if (R == 0) return TENRSO_OK;
   if (rows > GRAM_TALL) return gram_tall(ctx, A, rows, R, G, stream ? stream : ctx->stream);
-  hipLaunchKernelGGL(gram_rows_kernel, dim3((unsigned)R), dim3(1024), 0, stream ? stream : ctx->stream, A, rows,
-                     (int)R, G);
+  if (R <= SOLVE_RMAX)
+    hipLaunchKernelGGL(gram_rows_kernel<64>, dim3((unsigned)R), dim3(1024), 0, stream ? stream : ctx->stream, A, rows,
+                       (int)R, G);
+  else
+    hipLaunchKernelGGL(gram_rows_kernel<WIDE_NE>, dim3((unsigned)R), dim3(1024), 0, stream ? stream : ctx->stream, A,
+                       rows, (int)R, G);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -283,12 +302,167 @@ __global__ __launch_bounds__(JOB_THREADS) void als_job_kernel(AlsJob job) {
   als_job_run(job, lds);
 }
 
+// ---------------------------------------------------------------- wide-rank job
+// Ranks SOLVE_RMAX + 1 .. WIDE_NE: the steps of the job (the fit, Gamma_n plus
+// regularisation, its inverse with the ridge retry of cp/helpers.rs:477-490) as a
+// kernel of 1024 threads.  Thread t holds, in registers, column t % 128 of rows
+// t / 128 + 8 m (m = 0..15) of the 128 x 128 matrix, the identity outside R x R.
+// The inverse is Gauss-Jordan with scalar pivots in diagonal order: step k
+// publishes row k and column k (double-buffered in LDS, one barrier a step) and
+// every thread updates its entries from them.  For a symmetric matrix those
+// pivots are the ratios of consecutive leading principal minors, so all of them
+// are positive exactly when the matrix is positive definite: a pivot that is not
+// positive (or not finite) fails the attempt, as a pivot block that is not
+// positive definite fails it in the job of rank <= 64.
+constexpr int WIDE_THREADS = 1024;
+constexpr int WIDE_RPT = WIDE_NE * WIDE_NE / WIDE_THREADS;  // entries per thread
+
+__device__ __forceinline__ double wide_block_sum(double v, double* red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < WIDE_THREADS / 64; ++w) s += red[w];
+  return s;
+}
+
+// In-place inverse of the matrix held as described above; false if a pivot is
+// not positive.  Every thread returns at the same step.
+__device__ bool wide_gauss_jordan(double (&a)[WIDE_RPT], int R, double* rowb, double* colb) {
+  const int j = threadIdx.x & (WIDE_NE - 1), i0 = threadIdx.x >> 7;
+  for (int k = 0; k < R; ++k) {
+    double* rk = rowb + (k & 1) * WIDE_NE;
+    double* ck = colb + (k & 1) * WIDE_NE;
+    if ((k & 7) == i0) {
+      // row k = i0 + 8 (k >> 3), selected without a run-time register index
+      double v = 0.0;
+#pragma unroll
+      for (int m = 0; m < WIDE_RPT; ++m) v = m == (k >> 3) ? a[m] : v;
+      rk[j] = v;
+    }
+    if (j == k) {
+#pragma unroll
+      for (int m = 0; m < WIDE_RPT; ++m) ck[i0 + 8 * m] = a[m];
+    }
+    __syncthreads();
+    const double p = rk[k];
+    if (!(p > 0.0 && p <= DBL_MAX)) {
+      __syncthreads();  // every thread has read the buffers before they are reused
+      return false;
+    }
+    const double piv = 1.0 / p;
+    const double rj = rk[j] * piv;  // row k scaled by the pivot, column j
+#pragma unroll
+    for (int m = 0; m < WIDE_RPT; ++m) {
+      const int i = i0 + 8 * m;
+      const double cv = ck[i];
+      const double upd = fma(-cv, rj, a[m]), colv = -cv * piv;
+      a[m] = i == k ? (j == k ? piv : rj) : (j == k ? colv : upd);
+    }
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(WIDE_THREADS) void als_job_wide_kernel(AlsJob a) {
+  __shared__ double rowb[2 * WIDE_NE], colb[2 * WIDE_NE];
+  __shared__ double red0[WIDE_THREADS / 64], red1[WIDE_THREADS / 64];
+  const int R = a.R;
+  const int j = threadIdx.x & (WIDE_NE - 1), i0 = threadIdx.x >> 7;
+  double ipv = 0.0;
+  if (a.do_fit)
+    for (int b = threadIdx.x; b < a.nblk; b += WIDE_THREADS) ipv += a.ip_part[b];
+  // two passes over the Grams, each with one accumulator per entry in registers:
+  // the fit's hadamard of all k, then Gamma's hadamard of k != skip
+  auto hadamard = [&](double (&h)[WIDE_RPT], int skip) {
+#pragma unroll
+    for (int m = 0; m < WIDE_RPT; ++m) h[m] = 1.0;
+    for (int k = 0; k < a.nd; ++k) {
+      if (k == skip) continue;
+      const double* Gk = a.G + (i64)k * R * R + j;
+#pragma unroll
+      for (int m = 0; m < WIDE_RPT; ++m) {
+        const int i = i0 + 8 * m;
+        h[m] *= (i < R && j < R) ? Gk[i * R] : 0.0;
+      }
+    }
+  };
+  if (a.do_fit) {
+    // ||Xhat||^2 = sum_{r,s} prod_k G_k[r,s] (entries outside R x R are 0)
+    double cross[WIDE_RPT];
+    hadamard(cross, -1);
+    double rec = 0.0;
+#pragma unroll
+    for (int m = 0; m < WIDE_RPT; ++m) rec += cross[m];
+    const double recon = wide_block_sum(rec, red0);
+    const double ip = wide_block_sum(ipv, red1);
+    if (threadIdx.x == 0) {
+      double blk[JOB_SC_STAMP];
+#pragma unroll
+      for (int i = 0; i < JOB_SC_STAMP; ++i) blk[i] = a.scal[i];
+      const double nx = blk[JOB_SC_NORM_X];
+      const double err_sq = nx + recon - 2.0 * ip;
+      const double err = sqrt(err_sq > 0.0 ? err_sq : 0.0);
+      double fit = 1.0 - err / sqrt(nx);
+      fit = fit < 0.0 ? 0.0 : (fit > 1.0 ? 1.0 : fit);
+      blk[JOB_SC_FIT] = fit;
+      blk[JOB_SC_IP] = ip;
+      blk[JOB_SC_RECON] = recon;
+      a.scal[JOB_SC_FIT] = fit;
+      a.scal[JOB_SC_IP] = ip;
+      a.scal[JOB_SC_RECON] = recon;
+      if (a.host) {
+#pragma unroll
+        for (int i = 0; i < JOB_SC_STAMP; ++i) a.host[i] = blk[i];
+        __threadfence_system();
+        a.host[JOB_SC_STAMP] = a.stamp;
+      }
+    }
+  }
+  double gam[WIDE_RPT];
+  if (a.skip >= 0) hadamard(gam, a.skip);
+  if (a.gate) {
+    __syncthreads();  // every thread has its Grams
+    if (threadIdx.x == 0) __hip_atomic_store(a.gate, a.gate_stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (a.skip < 0) return;
+  // Gamma waits in Winv for the ridge retry (each thread rereads its own entries)
+  double w[WIDE_RPT];
+  auto form = [&](double ridge) {
+#pragma unroll
+    for (int m = 0; m < WIDE_RPT; ++m) {
+      const int i = i0 + 8 * m;
+      const double g = a.Winv[i * WIDE_NE + j];
+      w[m] = (i < R && j < R) ? g + job_reg(a, i, j, ridge) : (i == j ? 1.0 : 0.0);
+    }
+  };
+#pragma unroll
+  for (int m = 0; m < WIDE_RPT; ++m) a.Winv[(i0 + 8 * m) * WIDE_NE + j] = gam[m];
+  form(0.0);
+  bool ok = wide_gauss_jordan(w, R, rowb, colb);
+  int path = 0;
+  if (!ok) {
+    form((double)FLT_EPSILON * (double)(10 * R));
+    ok = wide_gauss_jordan(w, R, rowb, colb);
+    path = ok ? 1 : 2;
+  }
+#pragma unroll
+  for (int m = 0; m < WIDE_RPT; ++m) a.Winv[(i0 + 8 * m) * WIDE_NE + j] = ok ? w[m] : 0.0;
+  if (threadIdx.x == 0 && a.scal) {
+    const double prev = a.scal[a.status_slot];
+    a.scal[a.status_slot] = a.status_reset ? (double)path : fmax(prev, (double)path);
+  }
+}
+
 int als_job_launch(tenrso_ctx* ctx, const AlsJob& job) {
-  if (job.R > JOB_NE) {
-    set_error("rank %d exceeds the on-device normal-equation solve bound %d", job.R, JOB_NE);
+  if (job.R > WIDE_NE) {
+    set_error("rank %d exceeds the on-device normal-equation solve bound %d", job.R, WIDE_NE);
     return TENRSO_ERR_UNSUPPORTED;
   }
-  hipLaunchKernelGGL(als_job_kernel, dim3(1), dim3(JOB_THREADS), 0, ctx->stream, job);
+  if (job.R > JOB_NE)
+    hipLaunchKernelGGL(als_job_wide_kernel, dim3(1), dim3(WIDE_THREADS), 0, ctx->stream, job);
+  else
+    hipLaunchKernelGGL(als_job_kernel, dim3(1), dim3(JOB_THREADS), 0, ctx->stream, job);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
@@ -371,17 +545,86 @@ __global__ __launch_bounds__(256) void solve_rows_kernel(const float* __restrict
   (void)red;
 }
 
+// Ranks SOLVE_RMAX + 1 .. WIDE_NE: the same row solve with Winv of stride
+// WIDE_NE; 256 threads = 2 halves x 128 columns (each half sums half of the
+// split partials, then half of the r-sum), one row per workgroup.
+__global__ __launch_bounds__(256) void solve_rows_wide_kernel(const float* __restrict__ part, int splits,
+                                                              const double* __restrict__ M64, i64 rows, int R,
+                                                              const double* __restrict__ Winv, double l1, int nonneg,
+                                                              float* __restrict__ A, double* __restrict__ ip_part) {
+  __shared__ double sq[2 * WIDE_NE];
+  __shared__ double sM[WIDE_NE];
+  __shared__ double red[2];
+  const int c = threadIdx.x & (WIDE_NE - 1), h = threadIdx.x >> 7;
+  const i64 row = blockIdx.x;
+  const bool colok = c < R;
+  double m = 0.0;
+  if (colok) {
+    if (part) {
+      const i64 n = rows * R, off = row * R + c;
+      const int s0 = h * splits / 2, s1 = (h + 1) * splits / 2;
+      double a0 = 0, a1 = 0;
+      int sp = s0;
+      for (; sp + 2 <= s1; sp += 2) {
+        a0 += (double)part[(i64)sp * n + off];
+        a1 += (double)part[(i64)(sp + 1) * n + off];
+      }
+      if (sp < s1) a0 += (double)part[(i64)sp * n + off];
+      m = a0 + a1;
+    } else if (h == 0) {
+      m = M64[row * R + c];
+    }
+  }
+  sq[h * WIDE_NE + c] = m;
+  __syncthreads();
+  if (h == 0) sM[c] = sq[c] + sq[WIDE_NE + c];
+  __syncthreads();
+  double x = 0.0;
+  if (colok) {
+    const int r0 = h * R / 2, r1 = (h + 1) * R / 2;
+    double x0 = 0, x1 = 0;
+    int r = r0;
+    for (; r + 2 <= r1; r += 2) {
+      x0 += sM[r] * Winv[(i64)r * WIDE_NE + c];
+      x1 += sM[r + 1] * Winv[(i64)(r + 1) * WIDE_NE + c];
+    }
+    if (r < r1) x0 += sM[r] * Winv[(i64)r * WIDE_NE + c];
+    x = x0 + x1;
+  }
+  sq[h * WIDE_NE + c] = x;  // the reads of sq above finished before the last barrier
+  __syncthreads();
+  double ip = 0.0;
+  if (h == 0 && colok) {
+    double xv = sq[c] + sq[WIDE_NE + c];
+    if (l1 > 0.0) xv = xv > l1 ? xv - l1 : (xv < -l1 ? xv + l1 : 0.0);  // soft_threshold (helpers.rs:610-623)
+    if (nonneg) xv = xv > 0.0 ? xv : 0.0;                                  // core.rs:357-360
+    const float x32 = (float)xv;
+    A[row * R + c] = x32;
+    ip = sM[c] * (double)x32;
+  }
+  if (h == 0) {
+    ip = wave_sum(ip);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ip;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && ip_part) ip_part[blockIdx.x] = red[0] + red[1];
+}
+
 int solve_rows(tenrso_ctx* ctx, const float* part, int splits, const double* M, i64 rows, i64 R,
                const double* Winv, const RowEpilogue& epi, float* A, double* ip_part, int* nblk_out) {
   const unsigned nblk = cdiv(rows, SOLVE_RB);
   if (nblk_out) *nblk_out = (int)nblk;
   if (rows == 0) return TENRSO_OK;
-  if (R > SOLVE_RMAX) {
-    set_error("rank %lld exceeds the on-device solve bound %d", (long long)R, SOLVE_RMAX);
+  if (R > WIDE_NE) {
+    set_error("rank %lld exceeds the on-device solve bound %d", (long long)R, WIDE_NE);
     return TENRSO_ERR_UNSUPPORTED;
   }
-  hipLaunchKernelGGL(solve_rows_kernel, dim3(nblk), dim3(256), 0, ctx->stream, part, splits, M, rows, (int)R, Winv,
-                     epi.l1, epi.nonneg, A, ip_part);
+  if (R > SOLVE_RMAX)
+    hipLaunchKernelGGL(solve_rows_wide_kernel, dim3(nblk), dim3(256), 0, ctx->stream, part, splits, M, rows, (int)R,
+                       Winv, epi.l1, epi.nonneg, A, ip_part);
+  else
+    hipLaunchKernelGGL(solve_rows_kernel, dim3(nblk), dim3(256), 0, ctx->stream, part, splits, M, rows, (int)R, Winv,
+                       epi.l1, epi.nonneg, A, ip_part);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }

@@ -5,6 +5,9 @@
 namespace tenrso {
 
 constexpr int SOLVE_RMAX = 64;  // rank bound of the on-chip (LDS) normal-equation solve
+// Rank bound of the wide-rank ALS path (ranks SOLVE_RMAX + 1 .. WIDE_NE): the job
+// as its own kernel with the matrix in registers, Winv of stride WIDE_NE.
+constexpr int WIDE_NE = 128;
 
 // Regularisation applied to the Gram-Hadamard matrix (cp/core.rs:330-345).
 struct GramReg {

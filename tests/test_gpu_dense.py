@@ -476,6 +476,90 @@ def test_cp_als_ridge_retry_vs_oracle(shape, precision):
         assert rel(g[:, keep], r[:, keep]) < 1e-3
 
 
+# Ranks above 64 take the wide-rank job (Gamma_n^{-1} by scalar-pivot Gauss-Jordan
+# with the matrix in registers, as its own kernel), the wide row solve and the wide
+# Gram: rank 72 and the bound 128 on 3-way tensors, rank 100 on a 2-way one (Gamma
+# is a single Gram), and rank 66 beside a mode of 1100 rows (the chunked Gram, in
+# 64-row chunks at these ranks).  The 2-way rank stays below both sizes: at a rank
+# equal to the smaller size the model is exact after one sweep and the square
+# factor is fixed only up to the init, so rounding is not damped at all.
+ALS_WIDE = [((96, 80, 72), 72), ((130, 129, 128), 128), ((400, 300), 100), ((1100, 70, 66), 66)]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("shape,rank", ALS_WIDE)
+def test_cp_als_wide_rank_vs_oracle(shape, rank, precision):
+    """The bars and the oracle of test_cp_als_nway_vs_oracle, at ranks 66 to 128."""
+    test_cp_als_nway_vs_oracle(shape, rank, precision)
+
+
+@pytest.mark.parametrize("name,orc_kw,make", CONSTRAINTS[:5], ids=[c[0] for c in CONSTRAINTS[:5]])
+def test_cp_als_wide_rank_constrained_vs_oracle(name, orc_kw, make):
+    """The regularisation band and the row epilogue at rank 70 (the wide-rank job
+    forms the same Gamma_n + lambda D^T D or + l2 I, the wide row solve applies the
+    same soft threshold and clamp): 90x80x72 from the same init, 6 sweeps, fit
+    history within 1e-4 of the FP64 oracle, factors within 1e-3."""
+    print(f"[case] cp_als wide-rank constrained {name} 90x80x72 rank 70", flush=True)
+    rng = np.random.default_rng(37)
+    shape, R = (90, 80, 72), 70
+    A = [rng.random((n, R)) for n in shape]
+    X = (np.einsum("ir,jr,kr->ijk", *A) + 0.01 * rng.random(shape)).astype(np.float32)
+    init = [rng.random((n, R)).astype(np.float32) + 0.1 for n in shape]
+    cp = decomp.cp_als_with_init(dev(X), [dev(f) for f in init], 6, 0.0, constraints=make())
+    ref_F, info = oracle.cp_als(X, init, 6, 0.0, **orc_kw)
+    errs = [rel(g.cpu().numpy(), r) for g, r in zip(cp.factors, ref_F)]
+    dh = np.abs(np.array(cp.convergence.fit_history) - info["fit_history"]).max()
+    print(f"  gpu fit {cp.fit:.8f} oracle {info['fit']:.8f} paths {cp.solve_path_max} / {info['solve_path_max']} "
+          f"hist {dh:.1e} factor rel errs {['%.1e' % e for e in errs]}", flush=True)
+    assert cp.iters == info["iters"] and cp.solve_path_max == info["solve_path_max"]
+    assert dh <= 1e-4
+    assert max(errs) < 1e-3
+    if orc_kw.get("nonnegative"):
+        assert all(float(g.min()) >= 0.0 for g in cp.factors)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_cp_als_wide_rank_ridge_retry_vs_oracle(precision):
+    """The ridge retry in the wide-rank job: a zero column makes every Gamma_n
+    singular, the first elimination meets a zero pivot and the retry (Gamma + eps
+    10R I) succeeds, on the device and in the oracle (solve path 1 on both).  The
+    zero column stays exactly zero; the others match the oracle."""
+    shape, R = (80, 72, 70), 68
+    print(f"[case] cp_als wide-rank ridge retry {shape} rank {R} {precision}", flush=True)
+    rng = np.random.default_rng(41)
+    X = synth.cp_tensor(synth.normal_factors(shape, R, rng), noise=1e-2, rng=rng)
+    init = synth.init_factors(shape, R, seed=5)
+    for f in init:
+        f[:, 3] = 0.0
+    cp = decomp.cp_als_with_init(dev(X), [dev(f) for f in init], 5, 0.0, precision=precision)
+    lead = None if precision == "fp32" else bf16_round(X)
+    ref_F, info = oracle.cp_als(X, init, 5, 0.0, x_lead=lead)
+    keep = [c for c in range(R) if c != 3]
+    errs = [rel(g.cpu().numpy()[:, keep], r[:, keep]) for g, r in zip(cp.factors, ref_F)]
+    print(f"  fit {cp.fit:.8f} oracle {info['fit']:.8f} paths {cp.solve_path_max} / {info['solve_path_max']} "
+          f"factor rel errs {['%.1e' % e for e in errs]}", flush=True)
+    assert info["solve_path_max"] == 1 and cp.solve_path_max == 1
+    assert abs(cp.fit - info["fit"]) <= 1e-4
+    for g in cp.factors:
+        assert not g.cpu().numpy()[:, 3].any()
+    assert max(errs) < 1e-3
+
+
+def test_cp_als_rank_bounds():
+    """Rank 129 exceeds the device solve's bound of 128, and the orthogonal
+    constraint (one-workgroup Householder QR) supports rank <= 64: both are refused
+    with Unsupported before any launch."""
+    print("[case] cp_als rank bounds 130^3", flush=True)
+    import torch
+    X = torch.zeros((130, 130, 130), dtype=torch.float32, device="cuda")
+    init = [torch.zeros((130, 129), dtype=torch.float32, device="cuda") for _ in range(3)]
+    with pytest.raises(Exception, match="bound 128"):
+        decomp.cp_als_with_init(X, init, 2, 0.0)
+    init65 = [f[:, :65].contiguous() for f in init]
+    with pytest.raises(Exception, match="orthogonal constraint supports rank <= 64"):
+        decomp.cp_als_with_init(X, init65, 2, 0.0, constraints=decomp.CpConstraints.orthogonal_())
+
+
 def test_cp_als_bf16_start_gate_changes_timing_only(monkeypatch):
     """The start gate of the fused BF16 launch (the MTTKRP workgroups wait for the
     FP64 job's Gram loads) only delays work: factors and fit are the same bit for

@@ -225,11 +225,12 @@ def test_sparse_mttkrp_deterministic():
 
 
 @pytest.mark.parametrize("shape,nnz,R", [((60, 50, 40), 20000, 8), ((14, 12, 10, 9), 6000, 5), ((70, 50), 1500, 6),
-                                         ((2500, 30, 20), 30000, 8)])
+                                         ((2500, 30, 20), 30000, 8), ((90, 80, 75), 40000, 72)])
 def test_sparse_cp_als_vs_oracle(shape, nnz, R):
     """Sparse CP-ALS (3-, 4- and 2-way) against the FP64 oracle's dense ALS on the
     same tensor from the same initial factors: fit within 1e-4, same iterations.
-    (2500, 30, 20): a mode of more than 1024 rows, whose Gram is formed in chunks."""
+    (2500, 30, 20): a mode of more than 1024 rows, whose Gram is formed in chunks.
+    (90, 80, 75) rank 72: the wide-rank job and row solve behind the sparse MTTKRP."""
     print(f"[case] sparse cp_als {shape} nnz {nnz} rank {R}", flush=True)
     idx, vals = random_coo(shape, nnz, seed=7)
     sp = SparseTensor(idx, vals, shape)
