@@ -327,45 +327,117 @@ __device__ __forceinline__ double wide_block_sum(double v, double* red) {
   return s;
 }
 
-// In-place inverse of the matrix held as described above; false if a pivot is
-// not positive.  Every thread returns at the same step.
+// 4 x 4 inverse by scalar Gauss-Jordan in registers (every index fixed at compile
+// time); false if a pivot is not positive and finite.
+constexpr int WB = 4;  // pivots per elimination step of the wide-rank inverse
+__device__ __forceinline__ bool wide_inv4(double (&P)[WB][WB]) {
+  bool ok = true;
+#pragma unroll
+  for (int k = 0; k < WB; ++k) {
+    const double p = P[k][k];
+    ok = ok && p > 0.0 && p <= DBL_MAX;
+    const double piv = job_recip(p);
+    double rj[WB], cc[WB];
+#pragma unroll
+    for (int q = 0; q < WB; ++q) {
+      rj[q] = q == k ? 0.0 : P[k][q] * piv;
+      cc[q] = P[q][k];
+    }
+#pragma unroll
+    for (int r = 0; r < WB; ++r)
+#pragma unroll
+      for (int q = 0; q < WB; ++q) P[r][q] = fma(-cc[r], rj[q], P[r][q]);
+#pragma unroll
+    for (int q = 0; q < WB; ++q) {
+      if (q != k) {
+        P[k][q] = rj[q];
+        P[q][k] = -cc[q] * piv;
+      }
+    }
+    P[k][k] = piv;
+  }
+  return ok;
+}
+
+// In-place inverse of the matrix held as described above; false if a pivot
+// block is not positive definite.  Every thread returns at the same step.  Block
+// Gauss-Jordan with 4 x 4 pivot blocks J = {k0..k0+3} (the order padded to a
+// multiple of 4 with the identity), one barrier a step.  With P^-1 = A[J][J]^-1
+// and W = P^-1 A[J][:] (taken as 0 on the columns J):
+//   A[i][j] -= sum_q A[i][J_q] W[q][j]    every entry (columns J stay as they were),
+//   A[J][j]  = W[:, j]                    rows J (the four waves' rows holding them),
+//   A[i][J]  = -A[i][J] P^-1, A[J][J] = P^-1   columns J (four threads per row group).
+// Every thread inverts P itself from the published rows.  The pivot blocks are
+// positive definite exactly when the matrix is (their leading minors are ratios
+// of the matrix's), so the scalar pivots inside them decide the ridge retry.
+// Rows J are published as rk[r][j]; columns J as ck[((i % 8) * 16 + i / 8) * 4 + q],
+// each thread's 64 values contiguous.
 __device__ bool wide_gauss_jordan(double (&a)[WIDE_RPT], int R, double* rowb, double* colb) {
   const int j = threadIdx.x & (WIDE_NE - 1), i0 = threadIdx.x >> 7;
-  for (int k = 0; k < R; ++k) {
-    double* rk = rowb + (k & 1) * WIDE_NE;
-    double* ck = colb + (k & 1) * WIDE_NE;
-    if ((k & 7) == i0) {
-      // row k = i0 + 8 (k >> 3), selected without a run-time register index
+  const int Rp = (R + WB - 1) / WB * WB;
+  for (int k0 = 0; k0 < Rp; k0 += WB) {
+    double* rk = rowb + ((k0 / WB) & 1) * WB * WIDE_NE;
+    double* ck = colb + ((k0 / WB) & 1) * WB * WIDE_NE;
+    const int mk = k0 >> 3;          // rows k0..k0+3 are entry mk of their threads
+    const int r_own = i0 - (k0 & 7);
+    const bool row_owner = r_own >= 0 && r_own < WB;  // uniform per wave
+    const int q_own = j - k0;
+    const bool col_owner = q_own >= 0 && q_own < WB;
+    if (row_owner) {
+      // row k0 + r_own, selected without a run-time register index
       double v = 0.0;
 #pragma unroll
-      for (int m = 0; m < WIDE_RPT; ++m) v = m == (k >> 3) ? a[m] : v;
-      rk[j] = v;
+      for (int m = 0; m < WIDE_RPT; ++m) v = m == mk ? a[m] : v;
+      rk[r_own * WIDE_NE + j] = v;
     }
-    if (j == k) {
+    if (col_owner) {
 #pragma unroll
-      for (int m = 0; m < WIDE_RPT; ++m) ck[i0 + 8 * m] = a[m];
+      for (int m = 0; m < WIDE_RPT; ++m) ck[(i0 * WIDE_RPT + m) * WB + q_own] = a[m];
     }
     __syncthreads();
-    const double p = rk[k];
-    if (!(p > 0.0 && p <= DBL_MAX)) {
+    double P[WB][WB], rr[WB];
+#pragma unroll
+    for (int r = 0; r < WB; ++r) {
+#pragma unroll
+      for (int q = 0; q < WB; ++q) P[r][q] = rk[r * WIDE_NE + k0 + q];
+      rr[r] = rk[r * WIDE_NE + j];
+    }
+    if (!wide_inv4(P)) {
       __syncthreads();  // every thread has read the buffers before they are reused
       return false;
     }
-    const double piv = 1.0 / p;
-    const double rj = rk[j] * piv;  // row k scaled by the pivot, column j
+    // one pass of four FMAs per entry: columns J start from 0 and take column
+    // q_own of P^-1 (A[i][J] -> -A[i][J] P^-1), the others start from A[i][j] and
+    // take W[:, j] = P^-1 A[J][j]
+    double X[WB];
+#pragma unroll
+    for (int q = 0; q < WB; ++q) {
+      double w = 0.0;
+#pragma unroll
+      for (int r = 0; r < WB; ++r) w = fma(P[q][r], rr[r], w);
+      const double pq = q_own == 0 ? P[q][0] : (q_own == 1 ? P[q][1] : (q_own == 2 ? P[q][2] : P[q][3]));
+      X[q] = col_owner ? pq : w;
+    }
+    const double* cl = ck + i0 * WIDE_RPT * WB;
 #pragma unroll
     for (int m = 0; m < WIDE_RPT; ++m) {
-      const int i = i0 + 8 * m;
-      const double cv = ck[i];
-      const double upd = fma(-cv, rj, a[m]), colv = -cv * piv;
-      a[m] = i == k ? (j == k ? piv : rj) : (j == k ? colv : upd);
+      double v = col_owner ? 0.0 : a[m];
+#pragma unroll
+      for (int q = 0; q < WB; ++q) v = fma(-cl[m * WB + q], X[q], v);
+      a[m] = v;
+    }
+    if (row_owner) {
+      // rows J: W[r_own][j], or P^-1[r_own][q_own] on the columns J
+      const double xo = r_own == 0 ? X[0] : (r_own == 1 ? X[1] : (r_own == 2 ? X[2] : X[3]));
+#pragma unroll
+      for (int m = 0; m < WIDE_RPT; ++m) a[m] = m == mk ? xo : a[m];
     }
   }
   return true;
 }
 
 __global__ __launch_bounds__(WIDE_THREADS) void als_job_wide_kernel(AlsJob a) {
-  __shared__ double rowb[2 * WIDE_NE], colb[2 * WIDE_NE];
+  __shared__ double rowb[2 * WB * WIDE_NE], colb[2 * WB * WIDE_NE];
   __shared__ double red0[WIDE_THREADS / 64], red1[WIDE_THREADS / 64];
   const int R = a.R;
   const int j = threadIdx.x & (WIDE_NE - 1), i0 = threadIdx.x >> 7;
