@@ -189,6 +189,19 @@ def extra_configs(with_cpu):
     out["dense_250_r64_ragged"] = cr
     del Xd
 
+    # 256^3 rank 128: above rank 64 the normal-equation inverse runs as the
+    # wide-rank job (its own 1024-thread kernel, 4x4 pivot blocks)
+    X, _, _, init = synth.dense_256_r64(rank=128)
+    Xd = torch.from_numpy(X).to(dev)
+    Fd = [torch.from_numpy(f).to(dev) for f in init]
+    cw = {"shape": [256, 256, 256], "rank": 128}
+    for prec in ("fp32", "bf16"):
+        decomp.cp_als_with_init(Xd, Fd, 2, 0.0, precision=prec)
+        cp, ms = _dev_timed(lambda: decomp.cp_als_with_init(Xd, Fd, 10, 0.0, precision=prec))
+        cw[prec] = {"s_per_iter": ms / 1e4, "fit": cp.fit}
+    out["dense_256_r128_wide"] = cw
+    del Xd
+
     # out-of-core MTTKRP: a 512^3 fp32 host tensor (512 MiB, pinned) streamed in
     # mode-0 slabs through a 256 MiB device budget, rank 32
     g = torch.Generator().manual_seed(11)
