@@ -70,6 +70,11 @@ struct tenrso_ctx {
   int occ_val[32] = {};
   int occ_n = 0;
   void* blas = nullptr;              // rocBLAS handle of the n-mode products, made on first use
+  // side stream of the wide-rank ALS job (made on first use): the job overlaps the
+  // MTTKRP launched after it, and the next row solve joins it
+  hipStream_t job_stream = nullptr;
+  hipEvent_t job_fork = nullptr, job_join = nullptr;
+  bool job_join_pending = false;
 };
 
 namespace tenrso {

@@ -121,11 +121,15 @@ int tenrso_ctx_destroy(tenrso_ctx* c) {
   if (!c) return TENRSO_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->job_stream) (void)hipStreamSynchronize(c->job_stream);  // before its buffers go
   tenrso::blas_destroy(c);
   for (auto& b : c->bufs)
     if (b.ptr) (void)hipFree(b.ptr);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->job_stream) (void)hipStreamDestroy(c->job_stream);
+  if (c->job_fork) (void)hipEventDestroy(c->job_fork);
+  if (c->job_join) (void)hipEventDestroy(c->job_join);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -283,6 +283,7 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
       }
     }
   }
+  TENRSO_TRY(job_join(ctx));  // a wide-rank job forked for a solve that an error skipped
   TENRSO_HIP_TRY(hipStreamSynchronize(ctx->stream));  // drains a speculative sweep
   if (status != TENRSO_OK) return status;
 

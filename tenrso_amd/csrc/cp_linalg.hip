@@ -526,11 +526,37 @@ __global__ __launch_bounds__(WIDE_THREADS) void als_job_wide_kernel(AlsJob a) {
   }
 }
 
+int job_join(tenrso_ctx* ctx) {
+  if (!ctx->job_join_pending) return TENRSO_OK;
+  ctx->job_join_pending = false;
+  TENRSO_HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->job_join, 0));
+  return TENRSO_OK;
+}
+
 int als_job_launch(tenrso_ctx* ctx, const AlsJob& job) {
   if (job.R > WIDE_NE) {
     set_error("rank %d exceeds the on-device normal-equation solve bound %d", job.R, WIDE_NE);
     return TENRSO_ERR_UNSUPPORTED;
   }
+  if (job.R > JOB_NE && job.skip >= 0) {
+    // The wide-rank job with a row solve behind it runs on the side stream, after
+    // everything queued so far, beside the MTTKRP queued after it (the two are
+    // independent); solve_rows joins it before reading Winv.
+    if (!ctx->job_stream) {
+      TENRSO_HIP_TRY(hipStreamCreateWithFlags(&ctx->job_stream, hipStreamNonBlocking));
+      TENRSO_HIP_TRY(hipEventCreateWithFlags(&ctx->job_fork, hipEventDisableTiming));
+      TENRSO_HIP_TRY(hipEventCreateWithFlags(&ctx->job_join, hipEventDisableTiming));
+    }
+    TENRSO_TRY(job_join(ctx));
+    TENRSO_HIP_TRY(hipEventRecord(ctx->job_fork, ctx->stream));
+    TENRSO_HIP_TRY(hipStreamWaitEvent(ctx->job_stream, ctx->job_fork, 0));
+    hipLaunchKernelGGL(als_job_wide_kernel, dim3(1), dim3(WIDE_THREADS), 0, ctx->job_stream, job);
+    TENRSO_HIP_TRY(hipGetLastError());
+    TENRSO_HIP_TRY(hipEventRecord(ctx->job_join, ctx->job_stream));
+    ctx->job_join_pending = true;
+    return TENRSO_OK;
+  }
+  TENRSO_TRY(job_join(ctx));
   if (job.R > JOB_NE)
     hipLaunchKernelGGL(als_job_wide_kernel, dim3(1), dim3(WIDE_THREADS), 0, ctx->stream, job);
   else
@@ -691,6 +717,7 @@ int solve_rows(tenrso_ctx* ctx, const float* part, int splits, const double* M, 
     set_error("rank %lld exceeds the on-device solve bound %d", (long long)R, WIDE_NE);
     return TENRSO_ERR_UNSUPPORTED;
   }
+  TENRSO_TRY(job_join(ctx));  // Winv of a wide-rank job on the side stream
   if (R > SOLVE_RMAX)
     hipLaunchKernelGGL(solve_rows_wide_kernel, dim3(nblk), dim3(256), 0, ctx->stream, part, splits, M, rows, (int)R,
                        Winv, epi.l1, epi.nonneg, A, ip_part);

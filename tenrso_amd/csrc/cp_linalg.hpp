@@ -53,6 +53,9 @@ int orthonormalize_rows(tenrso_ctx* ctx, float* A, i64 rows, i64 R);
 int inner_rows(tenrso_ctx* ctx, const double* M, const float* A, i64 rows, i64 R, double* ip_part, int* nblk_out);
 // x = max(x, 0) on an fp32 array.
 int clamp_nonneg(tenrso_ctx* ctx, float* A, i64 n);
+// Makes the context stream wait for a wide-rank job still pending on the side
+// stream (no-op otherwise).
+int job_join(tenrso_ctx* ctx);
 // fill an fp64 array
 int fill_f64(tenrso_ctx* ctx, double* p, i64 n, double v);
 
