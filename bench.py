@@ -121,6 +121,24 @@ def _dev_timed(fn):
     return out, e0.elapsed_time(e1)
 
 
+def wide_rank_config():
+    """256^3 rank 128, 10 ALS iterations per precision: above rank 64 the
+    normal-equation inverse runs as the wide-rank job (its own 1024-thread kernel,
+    4x4 pivot blocks, on a side stream beside the MTTKRP)."""
+    import torch
+    from tenrso_amd import decomp, synth
+    dev = torch.device("cuda:0")
+    X, _, _, init = synth.dense_256_r64(rank=128)
+    Xd = torch.from_numpy(X).to(dev)
+    Fd = [torch.from_numpy(f).to(dev) for f in init]
+    cw = {"shape": [256, 256, 256], "rank": 128}
+    for prec in ("fp32", "bf16"):
+        decomp.cp_als_with_init(Xd, Fd, 2, 0.0, precision=prec)
+        cp, ms = _dev_timed(lambda: decomp.cp_als_with_init(Xd, Fd, 10, 0.0, precision=prec))
+        cw[prec] = {"s_per_iter": ms / 1e4, "fit": cp.fit}
+    return cw
+
+
 def extra_configs(with_cpu):
     """The other workloads of the hot path, one GPU: the CPU-sized ALS case, the
     ill-conditioned 512x512x128 case, the 4-way reverse-mode case and the 20M-nnz
@@ -189,18 +207,7 @@ def extra_configs(with_cpu):
     out["dense_250_r64_ragged"] = cr
     del Xd
 
-    # 256^3 rank 128: above rank 64 the normal-equation inverse runs as the
-    # wide-rank job (its own 1024-thread kernel, 4x4 pivot blocks)
-    X, _, _, init = synth.dense_256_r64(rank=128)
-    Xd = torch.from_numpy(X).to(dev)
-    Fd = [torch.from_numpy(f).to(dev) for f in init]
-    cw = {"shape": [256, 256, 256], "rank": 128}
-    for prec in ("fp32", "bf16"):
-        decomp.cp_als_with_init(Xd, Fd, 2, 0.0, precision=prec)
-        cp, ms = _dev_timed(lambda: decomp.cp_als_with_init(Xd, Fd, 10, 0.0, precision=prec))
-        cw[prec] = {"s_per_iter": ms / 1e4, "fit": cp.fit}
-    out["dense_256_r128_wide"] = cw
-    del Xd
+    out["dense_256_r128_wide"] = wide_rank_config()
 
     # out-of-core MTTKRP: a 512^3 fp32 host tensor (512 MiB, pinned) streamed in
     # mode-0 slabs through a 256 MiB device budget, rank 32
@@ -360,6 +367,8 @@ def main():
     ap.add_argument("--pmc-probe-sparse", action="store_true",
                     help="no timing: a 256 MiB device copy and three MTTKRPs per mode of the 20M-nonzero sparse "
                          "tensor, for rocprofv3 --pmc runs")
+    ap.add_argument("--wide-only", action="store_true",
+                    help="only the 256^3 rank-128 ALS (the wide-rank path), printed as JSON, for rocprofv3 --stats runs")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the per-mode MTTKRP kernel timings (--reps launches each), for rocprofv3 --stats runs")
     args = ap.parse_args()
@@ -384,6 +393,9 @@ def main():
         return
     if args.pmc_probe_sparse:
         pmc_probe_sparse()
+        return
+    if args.wide_only:
+        print(json.dumps({"dense_256_r128_wide": wide_rank_config()}), flush=True)
         return
 
     def timed_als(precision, calls=TIMED_CALLS):
