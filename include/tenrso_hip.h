@@ -248,8 +248,8 @@ typedef struct tenrso_cp_result {
 
 /* X: device tensor; factors: device pointers, in = initial factors, out = result.
  * fit_history: host buffer of max_iters doubles, or NULL.  Rank <= 128 (ranks
- * 65..128 take the wide-rank inverse); the orthogonal constraint needs rank <= 64.
- * A larger rank returns TENRSO_ERR_UNSUPPORTED before any launch. */
+ * 65..128 take the wide-rank inverse); a larger rank returns
+ * TENRSO_ERR_UNSUPPORTED before any launch. */
 int tenrso_cp_als_f32(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape, float* const* factors,
                       const tenrso_cp_options* opts, tenrso_cp_result* result, double* fit_history);
 int tenrso_cp_als_f32_host(tenrso_ctx* ctx, const float* X, int ndim, const int64_t* shape,

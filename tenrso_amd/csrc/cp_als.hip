@@ -70,10 +70,6 @@ int cp_als_run(tenrso_ctx* ctx, AlsProblem& pb, float* const* F, const tenrso_cp
     set_error("rank %lld exceeds the on-device normal-equation solve bound %d", (long long)R, WIDE_NE);
     return TENRSO_ERR_UNSUPPORTED;
   }
-  if (o->orthogonal && R > SOLVE_RMAX) {
-    set_error("the orthogonal constraint supports rank <= %d on the device (rank %lld)", SOLVE_RMAX, (long long)R);
-    return TENRSO_ERR_UNSUPPORTED;
-  }
   TENRSO_TRY(set_device(ctx));
   i64 maxI = 0, ftot = 0;
   for (int k = 0; k < nd; ++k) {

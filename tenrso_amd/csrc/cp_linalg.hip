@@ -737,17 +737,20 @@ int solve_rows(tenrso_ctx* ctx, const float* part, int splits, const double* M, 
 // already zero below and on the diagonal takes no reflection (rank-deficient
 // factors get an orthonormal completion, as Householder QR gives).
 // W: 2 * rows * R doubles of scratch (the reduced matrix with the reflectors
-// below its diagonal, then Q).  Thread t works on column t % 64 and row slice
-// t / 64 (4 slices); partial dot products are added in slice order.
-__global__ __launch_bounds__(256) void householder_q_kernel(float* __restrict__ A, i64 rows, int R,
-                                                            double* __restrict__ W) {
-  __shared__ double red[4][NE];
-  __shared__ double dots[NE];
-  __shared__ double v0s[NE], vns[NE], sgn[NE];
-  const int tid = threadIdx.x, col = tid & 63, slice = tid >> 6;
+// below its diagonal, then Q).  Thread t works on column t % NC and row slice
+// t / NC (4 slices; NC = 64, or 128 above rank 64); partial dot products are
+// added in slice order.
+template <int NC>
+__global__ __launch_bounds__(4 * NC) void householder_q_kernel(float* __restrict__ A, i64 rows, int R,
+                                                               double* __restrict__ W) {
+  constexpr int NT = 4 * NC;  // 4 row slices x NC columns
+  __shared__ double red[4][NC];
+  __shared__ double dots[NC];
+  __shared__ double v0s[NC], vns[NC], sgn[NC];
+  const int tid = threadIdx.x, col = tid % NC, slice = tid / NC;
   double* Q = W + rows * R;
   const i64 n = rows * R;
-  for (i64 e = tid; e < n; e += 256) W[e] = (double)A[e];
+  for (i64 e = tid; e < n; e += NT) W[e] = (double)A[e];
   __syncthreads();
   const int kmax = (int)(rows < R ? rows : R);
   // row range of this slice within [lo, rows)
@@ -799,7 +802,7 @@ __global__ __launch_bounds__(256) void householder_q_kernel(float* __restrict__ 
     __syncthreads();
   }
   // Q = H_0 ... H_{kmax-1} applied to the first R columns of the identity
-  for (i64 e = tid; e < n; e += 256) Q[e] = (e / R == e % R) ? 1.0 : 0.0;
+  for (i64 e = tid; e < n; e += NT) Q[e] = (e / R == e % R) ? 1.0 : 0.0;
   __syncthreads();
   for (int k = kmax - 1; k >= 0; --k) {
     const double vn = vns[k], v0 = v0s[k];
@@ -826,21 +829,25 @@ __global__ __launch_bounds__(256) void householder_q_kernel(float* __restrict__ 
     }
     __syncthreads();
   }
-  for (i64 e = tid; e < n; e += 256) {
+  for (i64 e = tid; e < n; e += NT) {
     const int j = (int)(e % R);
     A[e] = (float)(Q[e] * (j < kmax ? sgn[j] : 1.0));
   }
 }
 
 int orthonormalize_rows(tenrso_ctx* ctx, float* A, i64 rows, i64 R) {
-  if (R > SOLVE_RMAX) {
-    set_error("rank %lld exceeds the on-device orthonormalisation bound %d", (long long)R, SOLVE_RMAX);
+  if (R > WIDE_NE) {
+    set_error("rank %lld exceeds the on-device orthonormalisation bound %d", (long long)R, WIDE_NE);
     return TENRSO_ERR_UNSUPPORTED;
   }
   if (rows == 0 || R == 0) return TENRSO_OK;
   void* pw = nullptr;
   TENRSO_TRY(ctx_buffer(ctx, SLOT_ORTHO, (size_t)2 * rows * R * sizeof(double), &pw));
-  hipLaunchKernelGGL(householder_q_kernel, dim3(1), dim3(256), 0, ctx->stream, A, rows, (int)R, (double*)pw);
+  if (R <= SOLVE_RMAX)
+    hipLaunchKernelGGL(householder_q_kernel<64>, dim3(1), dim3(256), 0, ctx->stream, A, rows, (int)R, (double*)pw);
+  else
+    hipLaunchKernelGGL(householder_q_kernel<WIDE_NE>, dim3(1), dim3(4 * WIDE_NE), 0, ctx->stream, A, rows, (int)R,
+                       (double*)pw);
   TENRSO_HIP_TRY(hipGetLastError());
   return TENRSO_OK;
 }
