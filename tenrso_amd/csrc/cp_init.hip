@@ -33,6 +33,8 @@
 
 namespace tenrso {
 
+constexpr int INIT_RMAX = 128;  // rank bound of the SVD-based initialisations (that of the ALS)
+
 namespace {
 
 constexpr int GT = 32;          // Gram tile
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(1024) void svd_init_finish_kernel(const double* __r
                                                                const float* __restrict__ rnd, float* __restrict__ F) {
   __shared__ double red[16];
   __shared__ double lev_total;
-  __shared__ double colv[64][4];  // Nnsvd: ||u+||^2, ||u-||^2, ||z+||^2, ||z-||^2 per column
+  __shared__ double colv[INIT_RMAX][4];  // Nnsvd: ||u+||^2, ||u-||^2, ||z+||^2, ||z-||^2 per column
   const int tid = threadIdx.x;
   auto block_sum = [&](double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -384,8 +386,8 @@ int cp_init_svd(tenrso_ctx* ctx, const float* X, int nd, const i64* shape, i64 R
     set_error("tensors with %d modes are not supported", nd);
     return TENRSO_ERR_UNSUPPORTED;
   }
-  if (R <= 0 || R > 64) {
-    set_error("rank %lld outside the on-device SVD initialisation bound 1..64", (long long)R);
+  if (R <= 0 || R > INIT_RMAX) {
+    set_error("rank %lld outside the on-device SVD initialisation bound 1..%d", (long long)R, INIT_RMAX);
     return TENRSO_ERR_UNSUPPORTED;
   }
   i64 total = 1;

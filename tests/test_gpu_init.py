@@ -38,7 +38,8 @@ def model(shape, rank, seed=0):
 
 
 @pytest.mark.parametrize("init,kind", KINDS)
-@pytest.mark.parametrize("shape,rank", [((40, 36, 30), 5), ((24, 20, 18, 16), 4), ((96, 64, 33), 8)])
+@pytest.mark.parametrize("shape,rank", [((40, 36, 30), 5), ((24, 20, 18, 16), 4), ((96, 64, 33), 8),
+                                        ((80, 76, 74), 72)])
 def test_svd_init_vs_oracle(init, kind, shape, rank):
     print(f"[case] {kind} init {shape} rank {rank}", flush=True)
     X = model(shape, rank)
